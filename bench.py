@@ -1,0 +1,155 @@
+"""Benchmark: periodic 3-D inviscid Burgers, WENO5-Z (p=2, eps=1e-40) with
+global Lax-Friedrichs splitting and SSP-RK3, 256^3 FP64 on one MI355X.
+
+A step is one full SSP-RK3 step (three fused stage launches) over the whole
+grid; inputs are resident in HBM before the timed region starts.  Prints one
+JSON line (the contract the harness reads).
+
+    python bench.py [--steps K] [--warmup W] [--n 256] [--scheme Z2|JS]
+                    [--no-cpu] [--gpus 1]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Gpts/s and % HBM roofline, WENO5-Z 256³ FP64; adjoint/forward cost; batched vs 1-by-1"
+HBM_PEAK_GBS = 8000.0
+# Compulsory HBM traffic of one SSP-RK3 step with the fused stages, FP64:
+# stage 1 reads u^n, writes u1 (16 B/pt); stages 2 and 3 read their input and
+# u^n and write (24 B/pt each) -> 64 B/pt/step, 64/3 B/pt per stage launch.
+BYTES_PER_PT_STEP = 64.0
+LAUNCHES_PER_STEP = 3
+
+
+def read_traffic(workload: str):
+    """HBM bytes per stage launch from the committed rocprofv3 --pmc summary."""
+    path = os.path.join(ROOT, "profiles", f"{workload}_pmc.md")
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(path).read())
+    return float(m.group(1)) if m else None
+
+
+def cpu_baseline(n: int, scheme: str, seed: int):
+    """The CPU oracle (OpenMP restatement of the reference routines) timed on a
+    bounded sample of the same workload: whole SSP-RK3 steps of the n^3 grid,
+    as many as fit in about 10 s (at most 20)."""
+    import numpy as np
+
+    import oracle as O
+    from multigpu_advectiondiffusion_amd import solvers
+
+    kw = {"Z2": O.Z2_GLOBAL, "JS": O.JS_GLOBAL}[scheme]
+    u = solvers.ic_fourier(n, seed=seed)
+    O.burgers_step(u[:8, :8, :8].copy(), 2.0 / 8, 0.4, 0.0, 10.0, **kw)  # load the library
+    steps, t = 0, 0.0
+    t0 = time.perf_counter()
+    while steps < 20 and (t < 10.0 or steps == 0):
+        O.burgers_step(u, 2.0 / n, 0.4, 0.0, 10.0, **kw)
+        steps += 1
+        t = time.perf_counter() - t0
+    assert np.isfinite(u).all()
+    return {
+        "value": round(n ** 3 * steps / t / 1e9, 5),
+        "unit": "Gpts/s",
+        "cores": O.num_threads(),
+        "kind": "port",
+        "sample": f"{steps} SSP-RK3 steps of the {n}^3 {scheme} periodic grid in {t:.2f} s",
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--scheme", default="Z2", choices=["Z2", "JS"])
+    ap.add_argument("--seed", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-graph", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); "
+              "the multi-GPU z-slab decomposition is a design, not a built path.", file=sys.stderr)
+        return 2
+
+    from multigpu_advectiondiffusion_amd import hip, solvers
+
+    n = args.n
+    s = solvers.PeriodicBurgers(n, scheme=args.scheme, cfl=0.4)
+    u0 = solvers.ic_fourier(n, seed=args.seed)
+    s.set_initial(u0, t0=0.0, tend=float("inf"))
+    graph = not args.no_graph
+    s.steps(args.warmup, graph=graph)
+    s.stream.synchronize()
+
+    e0, e1 = hip.Event(), hip.Event()
+    hip.synchronize()
+    e0.record(s.stream)
+    s.steps(args.steps, graph=graph)
+    e1.record(s.stream)
+    e1.synchronize()
+    hip.synchronize()
+    ms = e0.elapsed_ms(e1)
+    st = s.state()
+    assert st.steps == args.warmup + args.steps
+
+    npts = n ** 3
+    sec = ms / 1e3
+    gpts = npts * args.steps / sec / 1e9
+    launch_s = sec / (args.steps * LAUNCHES_PER_STEP)
+    bytes_per_launch = BYTES_PER_PT_STEP / LAUNCHES_PER_STEP * npts
+    achieved = bytes_per_launch / launch_s / 1e9
+    workload = f"burgers3d_weno5{args.scheme.lower()}_{n}"
+    out = {
+        "metric": METRIC,
+        "value": round(gpts, 4),
+        "unit": "Gpts/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: 0.5 + 8 seeded Fourier modes (SplitMix64 seed %d)" % args.seed,
+        "config": {
+            "workload": workload,
+            "grid": [n, n, n],
+            "scheme": {"Z2": "WENO5-Z p=2 eps=1e-40", "JS": "WENO5-JS eps=1e-6"}[args.scheme],
+            "flux_split": "global Lax-Friedrichs alpha=max|u| per stage",
+            "time": "SSP-RK3, dt=0.4*dx/max|u^n| computed on device",
+            "boundary": "periodic on [-1,1)^3",
+            "hipgraph": graph,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 2),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": read_traffic(workload),
+            "kernel": "k_burgers_stage (one launch per SSP-RK3 stage)",
+            "avg_launch_us": round(launch_s * 1e6, 2),
+        },
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(n, args.scheme, args.seed)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

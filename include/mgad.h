@@ -1,0 +1,213 @@
+/*
+ * mgad.h -- C ABI of the MI355X (gfx950) advection-diffusion stencil library.
+ *
+ * Two groups of entry points:
+ *
+ *  1. Drop-in replacements for the reference's "device wrappers", same names,
+ *     same argument lists and meaning (REAL = double, cudaStream_t ->
+ *     hipStream_t, CUDA dim3 -> HIP dim3, both three 32-bit unsigned ints).
+ *     They return void like the reference; failures are reported through the
+ *     sticky code of mgad_get_last_error() instead of printf + exit(-1).
+ *     The launch-geometry arguments are accepted and ignored: the library
+ *     always covers the whole update region with its own launch shape.
+ *
+ *  2. mgad_* extensions for the periodic global-Lax-Friedrichs Burgers path
+ *     (fused SSP-RK3 stages with an on-device time step), the 7-point heat
+ *     forward-Euler step, and host helpers.  They return an mgad_status.
+ *
+ * All device pointers are caller-owned (allocated with hipMalloc or any
+ * allocator of the same device); no entry point allocates device memory.
+ * Arrays are x-fastest: index o = i + nx*j + nx*ny*k.
+ */
+#ifndef MGAD_H
+#define MGAD_H
+
+#include <stddef.h>
+#include <stdint.h>
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Status codes                                                              */
+/* ------------------------------------------------------------------------ */
+typedef enum {
+    MGAD_OK = 0,
+    MGAD_ERR_INVALID_ARG = 1,   /* bad size, null pointer, bad stage number */
+    MGAD_ERR_UNSUPPORTED = 2,   /* grid shape not supported by the kernel set */
+    MGAD_ERR_HIP = 3,           /* a HIP runtime call or a launch failed */
+    MGAD_ERR_NONFINITE = 4      /* max|u| read back as NaN or Inf */
+} mgad_status;
+
+/* Returns the first error recorded since the previous call and clears it. */
+int mgad_get_last_error(void);
+const char *mgad_error_string(int code);
+/* Library version string. */
+const char *mgad_version(void);
+
+/* ------------------------------------------------------------------------ */
+/* 1. Reference drop-in entry points                                         */
+/* ------------------------------------------------------------------------ */
+
+/* Replaces CopyToConstantMemory, SingleGPU/Burgers3d_WENO5/Burgers.h:72 and
+ * SingleGPU/Diffusion3d_baselineCode/acoustics3d.h:57 (same signature in
+ * both): diffusion coefficients kx = C/(12 dx^2) used by Call_Diff_ and
+ * Call_Lu3d.  Process-global, set once before launches. */
+void CopyToConstantMemory(double kx, double ky, double kz);
+
+/* Replaces Call_Adv_x, SingleGPU/Burgers3d_WENO5/Burgers.h:73-74
+ * (kernel Compute_dF, kernels.cu:197-260): Lu = -(F_{i+1/2}-F_{i-1/2})/dx on
+ * rows 2<j<Ny-3, 2<k<Nz-3 for i = 2..Nx-4 (i = 2 with F_{3/2} := 0). */
+void Call_Adv_x(dim3 numBlocks, dim3 threadsPerBlock, hipStream_t aStream,
+                unsigned int Nx, unsigned int Ny, unsigned int Nz, double dx,
+                double *u, double *Lu);
+/* Replaces Call_Adv_y, Burgers.h:75-76 (Compute_dG, kernels.cu:265-327):
+ * Lu -= (G_{j+1/2}-G_{j-1/2})/dy on 2<i<Nx-3, 2<k<Nz-3, j = 2..Ny-4. */
+void Call_Adv_y(dim3 numBlocks, dim3 threadsPerBlock, hipStream_t aStream,
+                unsigned int Nx, unsigned int Ny, unsigned int Nz, double dy,
+                double *v, double *Lv);
+/* Replaces Call_Adv_z, Burgers.h:77-78 (Compute_dH, kernels.cu:332-394):
+ * Lu -= (H_{k+1/2}-H_{k-1/2})/dz on 2<i<Nx-3, 2<j<Ny-3, k = 2..Nz-4. */
+void Call_Adv_z(dim3 numBlocks, dim3 threadsPerBlock, hipStream_t aStream,
+                unsigned int Nx, unsigned int Ny, unsigned int Nz, double dz,
+                double *w, double *Lw);
+/* Replaces Call_Diff_, Burgers.h:79-80 (Compute_Laplace_Async,
+ * kernels.cu:444-494): Lq += 13-point 4th-order Laplacian with the
+ * CopyToConstantMemory coefficients on 2<i<Nx-3, 2<j<Ny-3, k = 3..Nz-3. */
+void Call_Diff_(dim3 numBlocks, dim3 threadsPerBlock, hipStream_t aStream,
+                unsigned int Nx, unsigned int Ny, unsigned int Nz, double *q,
+                double *Lq);
+/* Replaces Call_sspRK, Burgers.h:81-82 (Compute_RK, kernels.cu:499-530):
+ * SSP-RK3 stage `step` (1..3) on 2<i<Nx-3, 2<j<Ny-3, 2<k<Nz-3. */
+void Call_sspRK(dim3 numBlocks, dim3 threadsPerBlock, hipStream_t aStream,
+                unsigned int Nx, unsigned int Ny, unsigned int Nz,
+                unsigned int step, double dt, double *u, double *uo,
+                double *Lu);
+/* Replaces Call_Lu3d, SingleGPU/Diffusion3d_baselineCode/acoustics3d.h:58
+ * (Compute_Laplace3d, kernels.cu:66-99): Lu = 13-point Laplacian on
+ * 1<i<nx-2, 1<j<ny-2, 1<k<nz-2. */
+void Call_Lu3d(dim3 numBlocks, dim3 threadsPerBlock, hipStream_t aStream,
+               unsigned int nx, unsigned int ny, unsigned int nz, double *u,
+               double *Lu);
+/* Replaces Call_RK3d, acoustics3d.h:59 (Compute_RK, kernels.cu:104-142):
+ * SSP-RK3 stage `step` on 1<i<nx-2, 1<j<ny-2, 1<k<nz-2. */
+void Call_RK3d(dim3 numBlocks, dim3 threadsPerBlock, hipStream_t aStream,
+               unsigned int step, unsigned int nx, unsigned int ny,
+               unsigned int nz, double dt, double *u, double *uo, double *Lu);
+
+/* ------------------------------------------------------------------------ */
+/* Scheme selection for the Call_Adv_* sweeps                                */
+/* ------------------------------------------------------------------------ */
+#define MGAD_WENO_Z     0  /* SingleGPU/Burgers3d_WENO5/kernels.cu:142-192 */
+#define MGAD_WENO_JS    1  /* MultiGPU/Burgers3d_Baseline/Kernels.cu:112-220 */
+#define MGAD_LF_LOCAL   0  /* alpha = |u_i|, SingleGPU/Burgers3d_WENO5/kernels.cu:224-238 */
+#define MGAD_LF_GLOBAL  1  /* alpha = given constant (max|u|) */
+
+/* Extra parameters of the Call_Adv_* sweeps (the reference fixes them at
+ * compile time, Burgers.h:20-28): WENO variant, WENO-Z power p (1 or 2),
+ * epsilon, LF mode and the global alpha.  Defaults reproduce the reference:
+ * WENO-Z, p = 1, eps = 1e-6, local LF.  Not thread-safe; set before launch. */
+int mgad_set_adv_scheme(int weno, int p, double eps, int lf, double alpha);
+
+/* Compute_Laplace3d_O2, SingleGPU/Diffusion3d_baselineCode/kernels.cu:37-64
+ * (defined there, never launched): Lu = 7-point Laplacian
+ * 12kx(ue-2uo+uw)+12ky(..)+12kz(..) on 0<i<nx-1 with the
+ * CopyToConstantMemory coefficients. */
+int mgad_lu3d_o2(hipStream_t stream, unsigned int nx, unsigned int ny,
+                 unsigned int nz, const double *u, double *Lu);
+
+/* ------------------------------------------------------------------------ */
+/* 2. Fused periodic Burgers path                                            */
+/* ------------------------------------------------------------------------ */
+
+/* Device-resident step control.  The caller allocates MGAD_CTRL_BYTES of
+ * device memory and passes it to every call of one run.  Fields (read with
+ * mgad_burgers_read_ctrl): current time t, dt of the last step, step count,
+ * and max|u| of the stage inputs for the global LF split and the CFL rule. */
+#define MGAD_CTRL_BYTES 256
+
+typedef struct {
+    double t;        /* time at the start of the next step */
+    double dt;       /* dt of the step in flight / last step (0 once done) */
+    double tend;     /* final time */
+    double cfl;      /* CFL number: dt = cfl*dx/max|u^n| */
+    double amax;     /* max|u^n| of the current solution */
+    uint64_t steps;  /* steps taken */
+} mgad_burgers_state;
+
+/* Scheme of the fused path. */
+typedef struct {
+    int weno;      /* MGAD_WENO_Z or MGAD_WENO_JS */
+    int p;         /* WENO-Z power: 1 or 2 */
+    double eps;    /* epsilon (JS 1e-6; Z p=2 uses 1e-40) */
+} mgad_scheme;
+
+/* Initialise ctrl for a run starting at time t0 from u (device, nx*ny*nz):
+ * computes max|u| on the device.  Asynchronous on stream. */
+int mgad_burgers_init(hipStream_t stream, unsigned int nx, unsigned int ny,
+                      unsigned int nz, const double *u, void *ctrl, double t0,
+                      double tend, double cfl);
+
+/* One SSP-RK3 stage on a periodic grid with global LF alpha = max|u_in|:
+ *   stage 1: u_out = u_n + dt*L(u_n)             (u_in == u_n)
+ *   stage 2: u_out = 0.75 u_n + 0.25 (u_in + dt*L(u_in))
+ *   stage 3: u_out = (u_n + 2 (u_in + dt*L(u_in)))/3
+ * with L(u) = -dF/dx - dG/dy - dH/dz (WENO5 face fluxes, the order of
+ * Call_Adv_x -> Call_Adv_y -> Call_Adv_z).  Stage 1 sets dt =
+ * cfl*dx/max|u_n| clipped to tend; the step is a no-op once t >= tend.
+ * u_out may alias u_n (stage 3 runs in place), never u_in. */
+int mgad_burgers_stage(hipStream_t stream, const mgad_scheme *scheme,
+                       unsigned int nx, unsigned int ny, unsigned int nz,
+                       double dx, double dy, double dz, int stage,
+                       const double *u_in, const double *u_n, double *u_out,
+                       void *ctrl);
+
+/* nsteps full SSP-RK3 steps: u (in/out) plus two scratch fields w1, w2 of
+ * nx*ny*nz doubles.  use_graph != 0 captures one step in a hipGraph and
+ * replays it.  Asynchronous on stream. */
+int mgad_burgers_steps(hipStream_t stream, const mgad_scheme *scheme,
+                       unsigned int nx, unsigned int ny, unsigned int nz,
+                       double dx, double dy, double dz, double *u, double *w1,
+                       double *w2, void *ctrl, int nsteps, int use_graph);
+
+/* Copies ctrl to the host (synchronises the stream). */
+int mgad_burgers_read_ctrl(hipStream_t stream, const void *ctrl,
+                           mgad_burgers_state *out);
+
+/* ------------------------------------------------------------------------ */
+/* Heat: 7-point forward Euler                                               */
+/* ------------------------------------------------------------------------ */
+
+/* nsteps forward-Euler steps u <- u + dt*L7(u) on the Dirichlet interior
+ * 0<i<n-1 (boundary nodes held), L7 as mgad_lu3d_o2 with coefficients
+ * kx, ky, kz (= C/(12 dx^2)).  u and w are two device fields; the result is
+ * left in u.  Asynchronous on stream. */
+int mgad_heat7_fe_steps(hipStream_t stream, unsigned int nx, unsigned int ny,
+                        unsigned int nz, double kx, double ky, double kz,
+                        double dt, double *u, double *w, int nsteps,
+                        int use_graph);
+
+/* ------------------------------------------------------------------------ */
+/* Host helpers (CPU only)                                                   */
+/* ------------------------------------------------------------------------ */
+
+/* u = 0.5 + sum_{m<nmodes} a_m sin(pi (kx_m x + ky_m y + kz_m z) + phi_m)
+ * on the periodic grid x_i = lo + i*(hi-lo)/n; integer wavenumbers uniform in
+ * 1..kmax, a_m uniform in [0, amp), phi_m uniform in [0, 2pi), drawn from
+ * SplitMix64(seed).  Host array. */
+int mgad_ic_fourier(double *u, unsigned int nx, unsigned int ny,
+                    unsigned int nz, double lo, double hi, uint64_t seed,
+                    int nmodes, int kmax, double amp, double offset);
+
+/* u = exp(-|x-c|^2/(2 sigma^2)) + noise*U(-1,1) on x_i = i/(n-1) in [0,1]^3,
+ * boundary nodes set to 0.  Host array. */
+int mgad_ic_gauss_noise(double *u, unsigned int nx, unsigned int ny,
+                        unsigned int nz, double sigma, double noise,
+                        uint64_t seed);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MGAD_H */

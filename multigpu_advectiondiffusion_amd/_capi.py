@@ -1,0 +1,118 @@
+"""ctypes binding of libmgad.so (the C ABI declared in include/mgad.h).
+
+The library is the only compute path: if libmgad.so is missing this module
+raises instead of falling back to anything else.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmgad.so")
+
+MGAD_OK, MGAD_ERR_INVALID_ARG, MGAD_ERR_UNSUPPORTED, MGAD_ERR_HIP, MGAD_ERR_NONFINITE = 0, 1, 2, 3, 4
+WENO_Z, WENO_JS = 0, 1
+LF_LOCAL, LF_GLOBAL = 0, 1
+CTRL_BYTES = 256
+
+# Every symbol include/mgad.h declares.
+EXPORTS = (
+    "mgad_get_last_error", "mgad_error_string", "mgad_version",
+    "CopyToConstantMemory", "Call_Adv_x", "Call_Adv_y", "Call_Adv_z", "Call_Diff_",
+    "Call_sspRK", "Call_Lu3d", "Call_RK3d",
+    "mgad_set_adv_scheme", "mgad_lu3d_o2",
+    "mgad_burgers_init", "mgad_burgers_stage", "mgad_burgers_steps", "mgad_burgers_read_ctrl",
+    "mgad_heat7_fe_steps", "mgad_ic_fourier", "mgad_ic_gauss_noise",
+)
+
+
+class Dim3(ctypes.Structure):
+    """HIP dim3: three 32-bit unsigned ints, passed by value."""
+    _fields_ = [("x", ctypes.c_uint32), ("y", ctypes.c_uint32), ("z", ctypes.c_uint32)]
+
+
+class Scheme(ctypes.Structure):
+    """mgad_scheme."""
+    _fields_ = [("weno", ctypes.c_int), ("p", ctypes.c_int), ("eps", ctypes.c_double)]
+
+
+class BurgersState(ctypes.Structure):
+    """mgad_burgers_state."""
+    _fields_ = [("t", ctypes.c_double), ("dt", ctypes.c_double), ("tend", ctypes.c_double),
+                ("cfl", ctypes.c_double), ("amax", ctypes.c_double), ("steps", ctypes.c_uint64)]
+
+
+class MgadError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise MgadError(f"{LIB_PATH} is not built: run __graft_entry__.build() "
+                            "(or python -m multigpu_advectiondiffusion_amd.build)")
+        L = ctypes.CDLL(LIB_PATH)
+        U, D, I, VP = ctypes.c_uint, ctypes.c_double, ctypes.c_int, ctypes.c_void_p
+        L.mgad_get_last_error.restype = I
+        L.mgad_error_string.restype = ctypes.c_char_p
+        L.mgad_error_string.argtypes = [I]
+        L.mgad_version.restype = ctypes.c_char_p
+        L.CopyToConstantMemory.argtypes = [D, D, D]
+        L.CopyToConstantMemory.restype = None
+        for name in ("Call_Adv_x", "Call_Adv_y", "Call_Adv_z"):
+            f = getattr(L, name)
+            f.argtypes = [Dim3, Dim3, VP, U, U, U, D, VP, VP]
+            f.restype = None
+        L.Call_Diff_.argtypes = [Dim3, Dim3, VP, U, U, U, VP, VP]
+        L.Call_Diff_.restype = None
+        L.Call_sspRK.argtypes = [Dim3, Dim3, VP, U, U, U, U, D, VP, VP, VP]
+        L.Call_sspRK.restype = None
+        L.Call_Lu3d.argtypes = [Dim3, Dim3, VP, U, U, U, VP, VP]
+        L.Call_Lu3d.restype = None
+        L.Call_RK3d.argtypes = [Dim3, Dim3, VP, U, U, U, U, D, VP, VP, VP]
+        L.Call_RK3d.restype = None
+        L.mgad_set_adv_scheme.argtypes = [I, I, D, I, D]
+        L.mgad_set_adv_scheme.restype = I
+        L.mgad_lu3d_o2.argtypes = [VP, U, U, U, VP, VP]
+        L.mgad_lu3d_o2.restype = I
+        L.mgad_burgers_init.argtypes = [VP, U, U, U, VP, VP, D, D, D]
+        L.mgad_burgers_init.restype = I
+        L.mgad_burgers_stage.argtypes = [VP, ctypes.POINTER(Scheme), U, U, U, D, D, D, I, VP, VP, VP, VP]
+        L.mgad_burgers_stage.restype = I
+        L.mgad_burgers_steps.argtypes = [VP, ctypes.POINTER(Scheme), U, U, U, D, D, D, VP, VP, VP, VP, I, I]
+        L.mgad_burgers_steps.restype = I
+        L.mgad_burgers_read_ctrl.argtypes = [VP, VP, ctypes.POINTER(BurgersState)]
+        L.mgad_burgers_read_ctrl.restype = I
+        L.mgad_heat7_fe_steps.argtypes = [VP, U, U, U, D, D, D, D, VP, VP, I, I]
+        L.mgad_heat7_fe_steps.restype = I
+        L.mgad_ic_fourier.argtypes = [VP, U, U, U, D, D, ctypes.c_uint64, I, I, D, D]
+        L.mgad_ic_fourier.restype = I
+        L.mgad_ic_gauss_noise.argtypes = [VP, U, U, U, D, D, ctypes.c_uint64]
+        L.mgad_ic_gauss_noise.restype = I
+        _lib = L
+    return _lib
+
+
+def error_string(code: int) -> str:
+    return lib().mgad_error_string(code).decode()
+
+
+def check(rc: int, what: str) -> None:
+    if rc != MGAD_OK:
+        raise MgadError(f"{what}: {error_string(rc)} (code {rc})")
+
+
+def last_error() -> int:
+    return lib().mgad_get_last_error()
+
+
+def raise_last(what: str) -> None:
+    """Raise if a void drop-in entry point recorded an error."""
+    rc = last_error()
+    if rc != MGAD_OK:
+        raise MgadError(f"{what}: {error_string(rc)} (code {rc})")

@@ -1,0 +1,118 @@
+"""Python mirror of the reference's device-wrapper interface.
+
+Same function names and argument order as the reference's extern "C" block
+(SingleGPU/Burgers3d_WENO5/Burgers.h:70-83 and
+SingleGPU/Diffusion3d_baselineCode/acoustics3d.h:55-60); buffers are
+DeviceArray objects (or raw device addresses), the stream a hip.Stream or
+None.  Each call goes straight to libmgad.so; like the reference the calls
+return nothing, and an argument error raises MgadError here (the C entry
+point records it in mgad_get_last_error()).
+"""
+from __future__ import annotations
+
+import ctypes
+
+from . import _capi
+from .hip import DeviceArray, Stream
+
+_DIM = _capi.Dim3(1, 1, 1)
+
+
+def _ptr(a):
+    if isinstance(a, DeviceArray):
+        return a.ptr
+    return ctypes.c_void_p(int(a))
+
+
+def _stream(s):
+    if s is None:
+        return None
+    if isinstance(s, Stream):
+        return s.handle
+    return ctypes.c_void_p(int(s))
+
+
+def _dim(d):
+    if isinstance(d, _capi.Dim3):
+        return d
+    if d is None:
+        return _DIM
+    d = tuple(d) + (1,) * (3 - len(tuple(d)))
+    return _capi.Dim3(*d)
+
+
+def CopyToConstantMemory(kx: float, ky: float, kz: float) -> None:
+    _capi.lib().CopyToConstantMemory(kx, ky, kz)
+
+
+def _adv(name, numBlocks, threadsPerBlock, aStream, Nx, Ny, Nz, d, u, Lu):
+    getattr(_capi.lib(), name)(_dim(numBlocks), _dim(threadsPerBlock), _stream(aStream), Nx, Ny, Nz, d,
+                               _ptr(u), _ptr(Lu))
+    _capi.raise_last(name)
+
+
+def Call_Adv_x(numBlocks, threadsPerBlock, aStream, Nx, Ny, Nz, dx, u, Lu) -> None:
+    _adv("Call_Adv_x", numBlocks, threadsPerBlock, aStream, Nx, Ny, Nz, dx, u, Lu)
+
+
+def Call_Adv_y(numBlocks, threadsPerBlock, aStream, Nx, Ny, Nz, dy, v, Lv) -> None:
+    _adv("Call_Adv_y", numBlocks, threadsPerBlock, aStream, Nx, Ny, Nz, dy, v, Lv)
+
+
+def Call_Adv_z(numBlocks, threadsPerBlock, aStream, Nx, Ny, Nz, dz, w, Lw) -> None:
+    _adv("Call_Adv_z", numBlocks, threadsPerBlock, aStream, Nx, Ny, Nz, dz, w, Lw)
+
+
+def Call_Diff_(numBlocks, threadsPerBlock, aStream, Nx, Ny, Nz, q, Lq) -> None:
+    _capi.lib().Call_Diff_(_dim(numBlocks), _dim(threadsPerBlock), _stream(aStream), Nx, Ny, Nz, _ptr(q), _ptr(Lq))
+    _capi.raise_last("Call_Diff_")
+
+
+def Call_sspRK(numBlocks, threadsPerBlock, aStream, Nx, Ny, Nz, step, dt, u, uo, Lu) -> None:
+    _capi.lib().Call_sspRK(_dim(numBlocks), _dim(threadsPerBlock), _stream(aStream), Nx, Ny, Nz, step, dt,
+                           _ptr(u), _ptr(uo), _ptr(Lu))
+    _capi.raise_last("Call_sspRK")
+
+
+def Call_Lu3d(numBlocks, threadsPerBlock, aStream, nx, ny, nz, u, Lu) -> None:
+    _capi.lib().Call_Lu3d(_dim(numBlocks), _dim(threadsPerBlock), _stream(aStream), nx, ny, nz, _ptr(u), _ptr(Lu))
+    _capi.raise_last("Call_Lu3d")
+
+
+def Call_RK3d(numBlocks, threadsPerBlock, aStream, step, nx, ny, nz, dt, u, uo, Lu) -> None:
+    _capi.lib().Call_RK3d(_dim(numBlocks), _dim(threadsPerBlock), _stream(aStream), step, nx, ny, nz, dt,
+                          _ptr(u), _ptr(uo), _ptr(Lu))
+    _capi.raise_last("Call_RK3d")
+
+
+def set_adv_scheme(weno: int = _capi.WENO_Z, p: int = 1, eps: float = 1e-6, lf: int = _capi.LF_LOCAL,
+                   alpha: float = 0.0) -> None:
+    """Extra Call_Adv_* parameters (defaults = the reference's compile-time choice)."""
+    _capi.check(_capi.lib().mgad_set_adv_scheme(weno, p, eps, lf, alpha), "mgad_set_adv_scheme")
+
+
+def lu3d_o2(u, Lu, nx, ny, nz, stream=None) -> None:
+    """Compute_Laplace3d_O2 (7-point) with the CopyToConstantMemory coefficients."""
+    _capi.check(_capi.lib().mgad_lu3d_o2(_stream(stream), nx, ny, nz, _ptr(u), _ptr(Lu)), "mgad_lu3d_o2")
+
+
+def burgers_reference_step(u, uo, Lu, Nx, Ny, Nz, dx, dy, dz, dt, stream=None, viscous=True) -> None:
+    """One time step in the reference driver's call order (SingleGPU/
+    Burgers3d_WENO5/main.cpp:139-149): uo <- u, then for each RK stage
+    Adv_x, Adv_y, Adv_z, Diff_, sspRK."""
+    uo.copy_from(u, stream)
+    for step in (1, 2, 3):
+        Call_Adv_x(None, None, stream, Nx, Ny, Nz, dx, u, Lu)
+        Call_Adv_y(None, None, stream, Nx, Ny, Nz, dy, u, Lu)
+        Call_Adv_z(None, None, stream, Nx, Ny, Nz, dz, u, Lu)
+        if viscous:
+            Call_Diff_(None, None, stream, Nx, Ny, Nz, u, Lu)
+        Call_sspRK(None, None, stream, Nx, Ny, Nz, step, dt, u, uo, Lu)
+
+
+def heat_reference_step(u, uo, Lu, nx, ny, nz, dt, stream=None) -> None:
+    """One time step of SingleGPU/Diffusion3d_baselineCode/main.cpp:108-121."""
+    uo.copy_from(u, stream)
+    for step in (1, 2, 3):
+        Call_Lu3d(None, None, stream, nx, ny, nz, u, Lu)
+        Call_RK3d(None, None, stream, step, nx, ny, nz, dt, u, uo, Lu)

@@ -1,0 +1,219 @@
+"""CPU oracle: ctypes bindings to oracle/liboracle.so.
+
+TEST INFRASTRUCTURE ONLY.  Imported by tests/, by __graft_entry__.smoke() and
+by bench.py's cpu_baseline leg, always as the checker.  The product package
+never imports it.
+
+Every function is a restatement of a reference routine (file:line cited in
+oracle.c) evaluated in IEEE binary64 with the reference's expression order.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+WENO_Z, WENO_JS, WENO_JS_B5, WENO_Z_B5S = 0, 1, 2, 3
+LF_LOCAL, LF_GLOBAL = 0, 1
+
+
+class Scheme(ctypes.Structure):
+    _fields_ = [
+        ("weno", ctypes.c_int),
+        ("p", ctypes.c_int),
+        ("eps", ctypes.c_double),
+        ("lf", ctypes.c_int),
+        ("alpha", ctypes.c_double),
+    ]
+
+
+def scheme(weno=WENO_Z, p=1, eps=1e-6, lf=LF_LOCAL, alpha=0.0) -> Scheme:
+    return Scheme(weno, p, eps, lf, alpha)
+
+
+# Reference defaults (B5/Burgers.h:26, B5/kernels.cu:142-192, local LF).
+REFERENCE = dict(weno=WENO_Z, p=1, eps=1e-6, lf=LF_LOCAL)
+# WENO5-JS, eps 1e-6, global LF (periodic Burgers, JS configuration).
+JS_GLOBAL = dict(weno=WENO_JS, p=2, eps=1e-6, lf=LF_GLOBAL)
+# WENO5-Z p=2, eps 1e-40, global LF (periodic Burgers, Z configuration).
+Z2_GLOBAL = dict(weno=WENO_Z, p=2, eps=1e-40, lf=LF_GLOBAL)
+
+
+def build(force: bool = False) -> str:
+    """Compile liboracle.so with the committed Makefile."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-s", "-C", _HERE, "-B" if force else "all"], check=True)
+    return _LIB_PATH
+
+
+_lib = None
+_D = ctypes.c_double
+_U = ctypes.c_uint
+_P = ctypes.POINTER(ctypes.c_double)
+_S = ctypes.POINTER(Scheme)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        L.orc_flux.restype = _D
+        L.orc_flux.argtypes = [_D]
+        L.orc_split.argtypes = [_D, _D, ctypes.c_int, _P, _P]
+        L.orc_face.restype = _D
+        L.orc_face.argtypes = [_P, _P, _S]
+        L.orc_weno_plus.restype = _D
+        L.orc_weno_plus.argtypes = [_P, _S]
+        L.orc_weno_minus.restype = _D
+        L.orc_weno_minus.argtypes = [_P, _S]
+        for name in ("orc_adv_x", "orc_adv_y", "orc_adv_z"):
+            getattr(L, name).argtypes = [_U, _U, _U, _D, _P, _P, _S]
+        L.orc_diff13_b5.argtypes = [_U, _U, _U, _D, _D, _D, _P, _P]
+        L.orc_ssprk_b5.argtypes = [_U, _U, _U, _U, _D, _P, _P, _P]
+        L.orc_lu3d_13.argtypes = [_U, _U, _U, _D, _D, _D, _P, _P]
+        L.orc_lu3d_o2.argtypes = [_U, _U, _U, _D, _D, _D, _P, _P]
+        L.orc_rk3d.argtypes = [_U, _U, _U, _U, _D, _P, _P, _P]
+        L.orc_maxabs.restype = _D
+        L.orc_maxabs.argtypes = [ctypes.c_uint64, _P]
+        L.orc_rhs_periodic.argtypes = [_U, _U, _U, _D, _D, _D, _P, _P, _S]
+        L.orc_rk_stage_all.argtypes = [ctypes.c_uint64, _U, _D, _P, _P, _P]
+        L.orc_burgers_step.restype = _D
+        L.orc_burgers_step.argtypes = [_U, _U, _U, _D, _D, _D, _D, _D, _D, _S, _P, _P]
+        L.orc_burgers_run.restype = ctypes.c_int
+        L.orc_burgers_run.argtypes = [_U, _U, _U, _D, _D, _D, _D, _P, _D, ctypes.c_int, _S, _P]
+        L.orc_heat7_fe.argtypes = [_U, _U, _U, _D, _D, _D, _D, ctypes.c_int, _P]
+        L.orc_matlab_heat3d.argtypes = [_U, _D, _D, _P, _P, _P, ctypes.POINTER(ctypes.c_int)]
+        L.orc_rhs_1d_periodic.argtypes = [_U, _D, _P, _P, _S]
+        L.orc_num_threads.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def _p(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_P)
+
+
+def _shape(u: np.ndarray):
+    """(nx, ny, nz) of an array stored x-fastest as u[k, j, i]."""
+    nz, ny, nx = u.shape
+    return nx, ny, nz
+
+
+def num_threads() -> int:
+    return lib().orc_num_threads()
+
+
+def flux(u: float) -> float:
+    return lib().orc_flux(u)
+
+
+def split(u: float, a: float = 0.0, lf: int = LF_LOCAL):
+    fp, fm = ctypes.c_double(), ctypes.c_double()
+    lib().orc_split(u, a, lf, ctypes.byref(fp), ctypes.byref(fm))
+    return fp.value, fm.value
+
+
+def face(v, w, **kw) -> float:
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    s = scheme(**kw)
+    return lib().orc_face(_p(v), _p(w), ctypes.byref(s))
+
+
+def weno_plus(v, **kw) -> float:
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    s = scheme(**kw)
+    return lib().orc_weno_plus(_p(v), ctypes.byref(s))
+
+
+def weno_minus(w, **kw) -> float:
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    s = scheme(**kw)
+    return lib().orc_weno_minus(_p(w), ctypes.byref(s))
+
+
+def adv(axis: int, u: np.ndarray, Lu: np.ndarray, d: float, **kw) -> None:
+    """Reference Call_Adv_{x,y,z} sweep (in place on Lu)."""
+    s = scheme(**kw)
+    fn = (lib().orc_adv_x, lib().orc_adv_y, lib().orc_adv_z)[axis]
+    fn(*_shape(u), d, _p(u), _p(Lu), ctypes.byref(s))
+
+
+def diff13_b5(u, Lu, kx, ky, kz) -> None:
+    lib().orc_diff13_b5(*_shape(u), kx, ky, kz, _p(u), _p(Lu))
+
+
+def ssprk_b5(u, uo, Lu, step, dt) -> None:
+    lib().orc_ssprk_b5(*_shape(u), step, dt, _p(u), _p(uo), _p(Lu))
+
+
+def lu3d_13(u, Lu, kx, ky, kz) -> None:
+    lib().orc_lu3d_13(*_shape(u), kx, ky, kz, _p(u), _p(Lu))
+
+
+def lu3d_o2(u, Lu, kx, ky, kz) -> None:
+    lib().orc_lu3d_o2(*_shape(u), kx, ky, kz, _p(u), _p(Lu))
+
+
+def rk3d(u, uo, Lu, step, dt) -> None:
+    nx, ny, nz = _shape(u)
+    lib().orc_rk3d(step, nx, ny, nz, dt, _p(u), _p(uo), _p(Lu))
+
+
+def maxabs(u) -> float:
+    return lib().orc_maxabs(u.size, _p(u))
+
+
+def rhs_periodic(u, d, **kw) -> np.ndarray:
+    dx, dy, dz = (d, d, d) if np.isscalar(d) else d
+    Lu = np.empty_like(u)
+    s = scheme(**kw)
+    lib().orc_rhs_periodic(*_shape(u), dx, dy, dz, _p(u), _p(Lu), ctypes.byref(s))
+    return Lu
+
+
+def rk_stage_all(u, uo, Lu, stage, dt) -> None:
+    lib().orc_rk_stage_all(u.size, stage, dt, _p(u), _p(uo), _p(Lu))
+
+
+def burgers_step(u, d, cfl, t, tend, **kw) -> float:
+    dx, dy, dz = (d, d, d) if np.isscalar(d) else d
+    work = np.empty(2 * u.size)
+    s = scheme(**kw)
+    return lib().orc_burgers_step(*_shape(u), dx, dy, dz, cfl, t, tend, ctypes.byref(s), _p(u), _p(work))
+
+
+def burgers_run(u, d, cfl, t, tend, nsteps=-1, **kw):
+    """Advance u in place; returns (steps, t)."""
+    dx, dy, dz = (d, d, d) if np.isscalar(d) else d
+    tt = ctypes.c_double(t)
+    s = scheme(**kw)
+    n = lib().orc_burgers_run(*_shape(u), dx, dy, dz, cfl, ctypes.byref(tt), tend, nsteps, ctypes.byref(s), _p(u))
+    return n, tt.value
+
+
+def heat7_fe(u, kx, ky, kz, dt, nsteps) -> None:
+    lib().orc_heat7_fe(*_shape(u), kx, ky, kz, dt, nsteps, _p(u))
+
+
+def matlab_heat3d(nc, tfinal=0.5, factor=0.9):
+    l1, l2, li = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+    it = ctypes.c_int()
+    lib().orc_matlab_heat3d(nc, tfinal, factor, ctypes.byref(l1), ctypes.byref(l2), ctypes.byref(li), ctypes.byref(it))
+    return l1.value, l2.value, li.value, it.value
+
+
+def rhs_1d_periodic(u, dx, **kw) -> np.ndarray:
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    Lu = np.empty_like(u)
+    s = scheme(**kw)
+    lib().orc_rhs_1d_periodic(u.size, dx, _p(u), _p(Lu), ctypes.byref(s))
+    return Lu

@@ -1,0 +1,589 @@
+/*
+ * oracle.c -- CPU restatement of the reference stencil routines
+ * (TEST INFRASTRUCTURE: only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg load this file's library, and only as the checker).
+ *
+ * Build: gcc -O2 -fopenmp -ffp-contract=off -fPIC -shared (oracle/Makefile).
+ * No -ffast-math, no contraction: every expression below is evaluated as the
+ * IEEE binary64 expression tree that the cited CUDA source spells out after
+ * macro expansion (e.g. C1312*(s)*(s) is ((13./12.)*s)*s, D0N*(x) is
+ * (1./10.)*x, 6*alphasum is 6.0*alphasum).
+ *
+ * What pins each function (tests/test_oracle_*.py):
+ *  - orc_flux, orc_split ............ closed forms: f+ + f- = f(u), f+ - f- = a*u,
+ *                                       local-LF u>=0 gives f+ = 3/4 u^2, f- = -1/4 u^2;
+ *                                       monotone split for a >= max|u|.
+ *  - orc_face (all variants) ........ constant data gives f(c); quadratic data makes
+ *                                       all candidate stencils agree; plus/minus mirror
+ *                                       symmetry; 5th-order convergence of the 1-D
+ *                                       periodic derivative (a swapped linear weight
+ *                                       or coefficient drops the order to <= 3);
+ *                                       brute force against a long-double restatement.
+ *  - orc_adv_x/y/z .................. constant field gives Lu = 0 on the written set;
+ *                                       x/y/z permutation of the data permutes Lu;
+ *                                       telescoping of the written differences.
+ *  - orc_rhs_periodic ............... sum(Lu) = 0 (periodic conservation); permutation
+ *                                       invariance; 1-D data reduces to orc_rhs_1d.
+ *  - orc_ssprk_b5, orc_rk3d,
+ *    orc_rk_stage_all ............... linear ODE u' = lambda*u gives the SSP-RK3
+ *                                       amplification 1 + z + z^2/2 + z^3/6.
+ *  - orc_lu3d_o2, orc_heat7_fe ...... Dirichlet sine eigenmodes decay by exactly
+ *                                       (1 - dt*lambda)^n; max principle.
+ *  - orc_lu3d_13, orc_diff13_b5 ..... agree with the MATLAB-order 13-point operator,
+ *                                       which orc_matlab_heat3d pins against the
+ *                                       reference's recorded accuracy log
+ *                                       (MLD/TestingAccuracy.log:66-72).
+ *  - orc_burgers_step/run ........... composition of the above; conservation of
+ *                                       sum(u) over a periodic run.
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* B5/Burgers.h:20-28 (same values in MB3/BurgersMPI.h:21-29) */
+#define D0N (1. / 10.)
+#define D1N (6. / 10.)
+#define D2N (3. / 10.)
+#define D0P (3. / 10.)
+#define D1P (6. / 10.)
+#define D2P (1. / 10.)
+#define C1312 (13. / 12.)
+#define C14 (1. / 4.)
+#define C1O6 (1. / 6.) /* B5S/Burgers.h:29 */
+
+int orc_num_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+/* B5/kernels.cu:37-40:  0.5*u*u  ==  (0.5*u)*u */
+double orc_flux(double u) { return 0.5 * u * u; }
+
+/* B5/kernels.cu:224-232 (local alpha = |u|);  MLB/WENO5resAdv1d.m:58-60 (global). */
+void orc_split(double u, double a, int lf, double *fp, double *fm) {
+    double f = orc_flux(u);
+    double au = (lf == ORC_LF_LOCAL) ? fabs(u) * u : a * u;
+    *fp = 0.5 * (f + au);
+    *fm = 0.5 * (f - au);
+}
+
+/* Smoothness indicators, B5/kernels.cu:157-159 (identical in MB3/Kernels.cu:176-178).
+ * a..e are the five stencil values (vmm, vm, v, vp, vpp). */
+static inline double beta0(double a, double b, double c) {
+    return C1312 * (a - 2 * b + c) * (a - 2 * b + c) + C14 * (a - 4 * b + 3 * c) * (a - 4 * b + 3 * c);
+}
+static inline double beta1(double b, double c, double d) {
+    return C1312 * (b - 2 * c + d) * (b - 2 * c + d) + C14 * (b - d) * (b - d);
+}
+static inline double beta2(double c, double d, double e) {
+    return C1312 * (c - 2 * d + e) * (c - 2 * d + e) + C14 * (3 * c - 4 * d + e) * (3 * c - 4 * d + e);
+}
+
+/* Candidate polynomials for the left-biased (f+) side, B5/kernels.cu:169-171. */
+static inline double pp0(const double *v) { return 2 * v[0] - 7 * v[1] + 11 * v[2]; }
+static inline double pp1(const double *v) { return -v[1] + 5 * v[2] + 2 * v[3]; }
+static inline double pp2(const double *v) { return 2 * v[2] + 5 * v[3] - v[4]; }
+/* Right-biased (f-) side, B5/kernels.cu:186-188. */
+static inline double pm0(const double *w) { return -w[0] + 5 * w[1] + 2 * w[2]; }
+static inline double pm1(const double *w) { return 2 * w[1] + 5 * w[2] - w[3]; }
+static inline double pm2(const double *w) { return 11 * w[2] - 7 * w[3] + 2 * w[4]; }
+
+static inline double zpow(double q, int p) { return p == 2 ? q * q : q; }
+
+/* f+ half of the face flux: v = f+ at nodes i-2..i+2. */
+double orc_weno_plus(const double *v, const orc_scheme *s) {
+    const double eps = s->eps;
+    double B0 = beta0(v[0], v[1], v[2]);
+    double B1 = beta1(v[1], v[2], v[3]);
+    double B2 = beta2(v[2], v[3], v[4]);
+    switch (s->weno) {
+    case ORC_WENO_Z: { /* B5/kernels.cu:162-171 */
+        double tau5 = fabs(B0 - B2);
+        double a0 = D0N * (1. + zpow(tau5 / (B0 + eps), s->p));
+        double a1 = D1N * (1. + zpow(tau5 / (B1 + eps), s->p));
+        double a2 = D2N * (1. + zpow(tau5 / (B2 + eps), s->p));
+        double alphasum = a0 + a1 + a2;
+        return (a0 * pp0(v) + a1 * pp1(v) + a2 * pp2(v)) / (6 * alphasum);
+    }
+    case ORC_WENO_Z_B5S: { /* B5S/kernels.cu:166-176 */
+        double tau5 = fabs(B0 - B2);
+        double a0 = D0N * (1. + zpow(tau5 / (B0 + eps), s->p));
+        double a1 = D1N * (1. + zpow(tau5 / (B1 + eps), s->p));
+        double a2 = D2N * (1. + zpow(tau5 / (B2 + eps), s->p));
+        double alphasum = 1. / (a0 + a1 + a2);
+        return (a0 * pp0(v) + a1 * pp1(v) + a2 * pp2(v)) * (C1O6 * alphasum);
+    }
+    case ORC_WENO_JS: { /* MB3/Kernels.cu:181-195 */
+        double a0 = D0N / ((eps + B0) * (eps + B0));
+        double a1 = D1N / ((eps + B1) * (eps + B1));
+        double a2 = D2N / ((eps + B2) * (eps + B2));
+        double alphasum = a0 + a1 + a2;
+        double w0 = a0 / alphasum, w1 = a1 / alphasum, w2 = a2 / alphasum;
+        return (w0 * pp0(v) + w1 * pp1(v) + w2 * pp2(v)) / 6;
+    }
+    default: { /* ORC_WENO_JS_B5, B5/kernels.cu:112-120 */
+        double a0 = D0N / ((eps + B0) * (eps + B0));
+        double a1 = D1N / ((eps + B1) * (eps + B1));
+        double a2 = D2N / ((eps + B2) * (eps + B2));
+        double alphasum = a0 + a1 + a2;
+        return (a0 * pp0(v) + a1 * pp1(v) + a2 * pp2(v)) / (6 * alphasum);
+    }
+    }
+}
+
+/* f- half of the face flux: w = f- at nodes i-1..i+3. */
+double orc_weno_minus(const double *w, const orc_scheme *s) {
+    const double eps = s->eps;
+    double B0 = beta0(w[0], w[1], w[2]);
+    double B1 = beta1(w[1], w[2], w[3]);
+    double B2 = beta2(w[2], w[3], w[4]);
+    switch (s->weno) {
+    case ORC_WENO_Z: { /* B5/kernels.cu:179-188 */
+        double tau5 = fabs(B0 - B2);
+        double a0 = D0P * (1. + zpow(tau5 / (B0 + eps), s->p));
+        double a1 = D1P * (1. + zpow(tau5 / (B1 + eps), s->p));
+        double a2 = D2P * (1. + zpow(tau5 / (B2 + eps), s->p));
+        double alphasum = a0 + a1 + a2;
+        return (a0 * pm0(w) + a1 * pm1(w) + a2 * pm2(w)) / (6 * alphasum);
+    }
+    case ORC_WENO_Z_B5S: { /* B5S/kernels.cu:189-200 */
+        double tau5 = fabs(B0 - B2);
+        double a0 = D0P * (1. + zpow(tau5 / (B0 + eps), s->p));
+        double a1 = D1P * (1. + zpow(tau5 / (B1 + eps), s->p));
+        double a2 = D2P * (1. + zpow(tau5 / (B2 + eps), s->p));
+        double alphasum = 1. / (a0 + a1 + a2);
+        return (a0 * pm0(w) + a1 * pm1(w) + a2 * pm2(w)) * (C1O6 * alphasum);
+    }
+    case ORC_WENO_JS: { /* MB3/Kernels.cu:202-216 */
+        double a0 = D0P / ((eps + B0) * (eps + B0));
+        double a1 = D1P / ((eps + B1) * (eps + B1));
+        double a2 = D2P / ((eps + B2) * (eps + B2));
+        double alphasum = a0 + a1 + a2;
+        double w0 = a0 / alphasum, w1 = a1 / alphasum, w2 = a2 / alphasum;
+        return (w0 * pm0(w) + w1 * pm1(w) + w2 * pm2(w)) / 6;
+    }
+    default: { /* ORC_WENO_JS_B5, B5/kernels.cu:128-136 */
+        double a0 = D0P / ((eps + B0) * (eps + B0));
+        double a1 = D1P / ((eps + B1) * (eps + B1));
+        double a2 = D2P / ((eps + B2) * (eps + B2));
+        double alphasum = a0 + a1 + a2;
+        return (a0 * pm0(w) + a1 * pm1(w) + a2 * pm2(w)) / (6 * alphasum);
+    }
+    }
+}
+
+/* Face flux F_{i+1/2} = f^+_{i+1/2} + f^-_{i+1/2}:
+ * B5/kernels.cu:168-191 (dflux = plus; dflux += minus), MB3/Kernels.cu:218 (hn+hp). */
+double orc_face(const double *v, const double *w, const orc_scheme *s) {
+    return orc_weno_plus(v, s) + orc_weno_minus(w, s);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Reference drop-in sweeps, frozen 3-cell ghosts.                           */
+/* ------------------------------------------------------------------------ */
+
+/* One line march shared by the three sweeps: B5/kernels.cu:224-258 (x),
+ * :291-325 (y), :358-392 (z).  base = linear index of node 0 of the line,
+ * st = stride between consecutive nodes, n = line length, d = spacing,
+ * mode 0 writes Lu = -(fu-fu_old)/d, mode 1 does Lu -= (fu-fu_old)/d. */
+static void march_line(const double *u, double *Lu, size_t base, size_t st,
+                       unsigned n, double d, int mode, const orc_scheme *s) {
+    double fp[5], fm[5], dummy;
+    double fu_old = 0;
+    /* f+ at nodes 0..3, f- at nodes 1..4 (B5/kernels.cu:224-232) */
+    for (int q = 0; q < 4; q++) orc_split(u[base + st * q], s->alpha, s->lf, &fp[q], &dummy);
+    for (int q = 0; q < 4; q++) orc_split(u[base + st * (q + 1)], s->alpha, s->lf, &dummy, &fm[q]);
+    for (unsigned i = 2; i < n - 3; i++) {
+        orc_split(u[base + st * (i + 2)], s->alpha, s->lf, &fp[4], &dummy);
+        orc_split(u[base + st * (i + 3)], s->alpha, s->lf, &dummy, &fm[4]);
+        double fu = orc_face(fp, fm, s);
+        size_t o = base + st * i;
+        if (mode == 0)
+            Lu[o] = -(fu - fu_old) / d;
+        else
+            Lu[o] -= (fu - fu_old) / d;
+        fu_old = fu;
+        for (int q = 0; q < 4; q++) { fp[q] = fp[q + 1]; fm[q] = fm[q + 1]; }
+    }
+}
+
+/* Compute_dF, B5/kernels.cu:197-260: rows 2<j<ny-3, 2<k<nz-3, i = 2..nx-4. */
+void orc_adv_x(unsigned nx, unsigned ny, unsigned nz, double dx, const double *u,
+               double *Lu, const orc_scheme *s) {
+    if (nx < 7 || ny < 7 || nz < 7) return;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (unsigned k = 3; k < nz - 3; k++)
+        for (unsigned j = 3; j < ny - 3; j++)
+            march_line(u, Lu, (size_t)nx * j + (size_t)nx * ny * k, 1, nx, dx, 0, s);
+}
+
+/* Compute_dG, B5/kernels.cu:265-327: columns 2<i<nx-3, 2<k<nz-3, j = 2..ny-4. */
+void orc_adv_y(unsigned nx, unsigned ny, unsigned nz, double dy, const double *u,
+               double *Lu, const orc_scheme *s) {
+    if (nx < 7 || ny < 7 || nz < 7) return;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (unsigned k = 3; k < nz - 3; k++)
+        for (unsigned i = 3; i < nx - 3; i++)
+            march_line(u, Lu, i + (size_t)nx * ny * k, nx, ny, dy, 1, s);
+}
+
+/* Compute_dH, B5/kernels.cu:332-394: columns 2<i<nx-3, 2<j<ny-3, k = 2..nz-4. */
+void orc_adv_z(unsigned nx, unsigned ny, unsigned nz, double dz, const double *u,
+               double *Lu, const orc_scheme *s) {
+    if (nx < 7 || ny < 7 || nz < 7) return;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (unsigned j = 3; j < ny - 3; j++)
+        for (unsigned i = 3; i < nx - 3; i++)
+            march_line(u, Lu, i + (size_t)nx * j, (size_t)nx * ny, nz, dz, 1, s);
+}
+
+/* Compute_Laplace_Async, B5/kernels.cu:444-494, launched by Call_Diff_
+ * (:560-565) with kstart=3, kstop=nz-2: Lu += 13-point operator on
+ * 2<i<nx-3, 2<j<ny-3 and planes k = 3..nz-3, each plane once. */
+void orc_diff13_b5(unsigned nx, unsigned ny, unsigned nz, double kx, double ky,
+                   double kz, const double *u, double *Lu) {
+    if (nx < 7 || ny < 7 || nz < 7) return;
+    const size_t px = nx, px2 = 2 * (size_t)nx, XY = (size_t)nx * ny, XY2 = 2 * XY;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (unsigned k = 3; k < nz - 2; k++)
+        for (unsigned j = 3; j < ny - 3; j++)
+            for (unsigned i = 3; i < nx - 3; i++) {
+                size_t o = i + px * j + XY * k;
+                double center = u[o];
+                double below2 = u[o - XY2], below = u[o - XY], above = u[o + XY], above2 = u[o + XY2];
+                Lu[o] += kx * (-u[o - 2] + 16 * u[o - 1] - 30 * center + 16 * u[o + 1] - u[o + 2]) +
+                         ky * (-u[o - px2] + 16 * u[o - px] - 30 * center + 16 * u[o + px] - u[o + px2]) +
+                         kz * (-below2 + 16 * below - 30 * center + 16 * above - above2);
+            }
+}
+
+/* One SSP-RK3 stage at one node: B5/kernels.cu:520-527 and D3/kernels.cu:129-139. */
+static inline double rk_node(unsigned step, double dt, double u, double uo, double Lu) {
+    switch (step) {
+    case 1: return uo + dt * (Lu);
+    case 2: return 0.75 * uo + 0.25 * (u + dt * (Lu));
+    case 3: return (uo + 2 * (u + dt * (Lu))) / 3;
+    default: return u;
+    }
+}
+
+/* Compute_RK, B5/kernels.cu:499-530: interior 2<i<nx-3 (3-cell ghosts frozen). */
+void orc_ssprk_b5(unsigned nx, unsigned ny, unsigned nz, unsigned step, double dt,
+                  double *u, const double *uo, const double *Lu) {
+    if (nx < 7 || ny < 7 || nz < 7) return;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (unsigned k = 3; k < nz - 3; k++)
+        for (unsigned j = 3; j < ny - 3; j++)
+            for (unsigned i = 3; i < nx - 3; i++) {
+                size_t o = i + (size_t)nx * j + (size_t)nx * ny * k;
+                u[o] = rk_node(step, dt, u[o], uo[o], Lu[o]);
+            }
+}
+
+/* Compute_Laplace3d, D3/kernels.cu:66-99: 13-point operator on 1<i<nx-2. */
+void orc_lu3d_13(unsigned nx, unsigned ny, unsigned nz, double kx, double ky,
+                 double kz, const double *u, double *Lu) {
+    if (nx < 5 || ny < 5 || nz < 5) return;
+    const size_t xy = (size_t)nx * ny;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (unsigned k = 2; k < nz - 2; k++)
+        for (unsigned j = 2; j < ny - 2; j++)
+            for (unsigned i = 2; i < nx - 2; i++) {
+                size_t o = i + (size_t)nx * j + xy * k;
+                size_t nn = o + nx + nx, n = o + nx, s = o - nx, ss = o - nx - nx;
+                size_t ee = o + 2, e = o + 1, w = o - 1, ww = o - 2;
+                size_t tt = o + xy + xy, t = o + xy, b = o - xy, bb = o - xy - xy;
+                Lu[o] = kx * (-u[ee] + 16 * u[e] - 30 * u[o] + 16 * u[w] - u[ww]) +
+                        ky * (-u[nn] + 16 * u[n] - 30 * u[o] + 16 * u[s] - u[ss]) +
+                        kz * (-u[tt] + 16 * u[t] - 30 * u[o] + 16 * u[b] - u[bb]);
+            }
+}
+
+/* Compute_Laplace3d_O2, D3/kernels.cu:37-64: 7-point operator on 0<i<nx-1. */
+void orc_lu3d_o2(unsigned nx, unsigned ny, unsigned nz, double kx, double ky,
+                 double kz, const double *u, double *Lu) {
+    if (nx < 3 || ny < 3 || nz < 3) return;
+    const size_t xy = (size_t)nx * ny;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (unsigned k = 1; k < nz - 1; k++)
+        for (unsigned j = 1; j < ny - 1; j++)
+            for (unsigned i = 1; i < nx - 1; i++) {
+                size_t o = i + (size_t)nx * j + xy * k;
+                size_t n = o + nx, s = o - nx, e = o + 1, w = o - 1, t = o + xy, b = o - xy;
+                Lu[o] = 12 * kx * (u[e] - 2 * u[o] + u[w]) +
+                        12 * ky * (u[n] - 2 * u[o] + u[s]) +
+                        12 * kz * (u[t] - 2 * u[o] + u[b]);
+            }
+}
+
+/* Compute_RK, D3/kernels.cu:104-142: interior 1<i<nx-2 (2-cell ring frozen). */
+void orc_rk3d(unsigned step, unsigned nx, unsigned ny, unsigned nz, double dt,
+              double *u, const double *uo, const double *Lu) {
+    if (nx < 5 || ny < 5 || nz < 5) return;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (unsigned k = 2; k < nz - 2; k++)
+        for (unsigned j = 2; j < ny - 2; j++)
+            for (unsigned i = 2; i < nx - 2; i++) {
+                size_t o = i + (size_t)nx * j + (size_t)nx * ny * k;
+                u[o] = rk_node(step, dt, u[o], uo[o], Lu[o]);
+            }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Periodic global-LF Burgers                                               */
+/* ------------------------------------------------------------------------ */
+
+double orc_maxabs(uint64_t n, const double *u) {
+    double m = 0.0;
+#pragma omp parallel for reduction(max : m) schedule(static)
+    for (int64_t q = 0; q < (int64_t)n; q++) {
+        double a = fabs(u[q]);
+        if (a > m) m = a;
+    }
+    return m;
+}
+
+static inline unsigned wrap(long i, unsigned n) {
+    long r = i % (long)n;
+    return (unsigned)(r < 0 ? r + (long)n : r);
+}
+
+/* Face F at (i+1/2) along axis `ax` for every node, periodic wrap.
+ * fp/fm are the split fluxes of every node. */
+static void faces_periodic(unsigned nx, unsigned ny, unsigned nz, int ax,
+                           const double *fp, const double *fm, double *F,
+                           const orc_scheme *s) {
+    const size_t xy = (size_t)nx * ny;
+    const unsigned n = ax == 0 ? nx : (ax == 1 ? ny : nz);
+#pragma omp parallel for collapse(2) schedule(static)
+    for (unsigned k = 0; k < nz; k++)
+        for (unsigned j = 0; j < ny; j++)
+            for (unsigned i = 0; i < nx; i++) {
+                unsigned c = ax == 0 ? i : (ax == 1 ? j : k);
+                double v[5], w[5];
+                for (int q = 0; q < 5; q++) {
+                    unsigned a = wrap((long)c - 2 + q, n); /* f+ at c-2..c+2 */
+                    unsigned b = wrap((long)c - 1 + q, n); /* f- at c-1..c+3 */
+                    size_t oa = ax == 0 ? a + (size_t)nx * j + xy * k
+                              : ax == 1 ? i + (size_t)nx * a + xy * k
+                                        : i + (size_t)nx * j + xy * a;
+                    size_t ob = ax == 0 ? b + (size_t)nx * j + xy * k
+                              : ax == 1 ? i + (size_t)nx * b + xy * k
+                                        : i + (size_t)nx * j + xy * b;
+                    v[q] = fp[oa];
+                    w[q] = fm[ob];
+                }
+                F[i + (size_t)nx * j + xy * k] = orc_face(v, w, s);
+            }
+}
+
+/* Periodic restatement of the Call_Adv_x -> Call_Adv_y -> Call_Adv_z sequence
+ * (B5/main.cpp:144-146): Lu = -(F_{i+1/2}-F_{i-1/2})/dx (overwrite, :244),
+ * then Lu -= dG/dy (:311), then Lu -= dH/dz (:378), every node updated. */
+void orc_rhs_periodic(unsigned nx, unsigned ny, unsigned nz, double dx, double dy,
+                      double dz, const double *u, double *Lu, const orc_scheme *s) {
+    const size_t N = (size_t)nx * ny * nz, xy = (size_t)nx * ny;
+    double *fp = (double *)malloc(sizeof(double) * N);
+    double *fm = (double *)malloc(sizeof(double) * N);
+    double *F = (double *)malloc(sizeof(double) * N);
+#pragma omp parallel for schedule(static)
+    for (int64_t q = 0; q < (int64_t)N; q++) orc_split(u[q], s->alpha, s->lf, &fp[q], &fm[q]);
+
+    faces_periodic(nx, ny, nz, 0, fp, fm, F, s);
+#pragma omp parallel for collapse(2) schedule(static)
+    for (unsigned k = 0; k < nz; k++)
+        for (unsigned j = 0; j < ny; j++)
+            for (unsigned i = 0; i < nx; i++) {
+                size_t o = i + (size_t)nx * j + xy * k;
+                size_t om = wrap((long)i - 1, nx) + (size_t)nx * j + xy * k;
+                Lu[o] = -(F[o] - F[om]) / dx;
+            }
+    faces_periodic(nx, ny, nz, 1, fp, fm, F, s);
+#pragma omp parallel for collapse(2) schedule(static)
+    for (unsigned k = 0; k < nz; k++)
+        for (unsigned j = 0; j < ny; j++)
+            for (unsigned i = 0; i < nx; i++) {
+                size_t o = i + (size_t)nx * j + xy * k;
+                size_t om = i + (size_t)nx * wrap((long)j - 1, ny) + xy * k;
+                Lu[o] -= (F[o] - F[om]) / dy;
+            }
+    faces_periodic(nx, ny, nz, 2, fp, fm, F, s);
+#pragma omp parallel for collapse(2) schedule(static)
+    for (unsigned k = 0; k < nz; k++)
+        for (unsigned j = 0; j < ny; j++)
+            for (unsigned i = 0; i < nx; i++) {
+                size_t o = i + (size_t)nx * j + xy * k;
+                size_t om = i + (size_t)nx * j + xy * wrap((long)k - 1, nz);
+                Lu[o] -= (F[o] - F[om]) / dz;
+            }
+    free(fp);
+    free(fm);
+    free(F);
+}
+
+/* Stage update on every node (periodic grid has no ghosts). */
+void orc_rk_stage_all(uint64_t n, unsigned stage, double dt, double *u,
+                      const double *uo, const double *Lu) {
+#pragma omp parallel for schedule(static)
+    for (int64_t q = 0; q < (int64_t)n; q++) u[q] = rk_node(stage, dt, u[q], uo[q], Lu[q]);
+}
+
+/* One SSP-RK3 step, MLB/LFWENO5FDM3d.m:70-95 with the CUDA sign convention
+ * (Lu = -div F, B5/main.cpp:143-149): dt = CFL*dx/max|u^n| clipped to tEnd
+ * (:71); each stage splits with alpha = max|u| of its own input
+ * (MLB/WENO5resAdv1d.m:58).  work holds 2*N doubles (uo, Lu). */
+double orc_burgers_step(unsigned nx, unsigned ny, unsigned nz, double dx, double dy,
+                        double dz, double cfl, double t, double tend,
+                        const orc_scheme *s0, double *u, double *work) {
+    const size_t N = (size_t)nx * ny * nz;
+    double *uo = work, *Lu = work + N;
+    orc_scheme s = *s0;
+    double amax = orc_maxabs(N, u);
+    double dt = cfl * dx / amax;
+    if (t + dt > tend) dt = tend - t;
+    memcpy(uo, u, sizeof(double) * N);
+    for (unsigned stage = 1; stage <= 3; stage++) {
+        if (s.lf == ORC_LF_GLOBAL) s.alpha = stage == 1 ? amax : orc_maxabs(N, u);
+        orc_rhs_periodic(nx, ny, nz, dx, dy, dz, u, Lu, &s);
+        orc_rk_stage_all(N, stage, dt, u, uo, Lu);
+    }
+    return dt;
+}
+
+int orc_burgers_run(unsigned nx, unsigned ny, unsigned nz, double dx, double dy,
+                    double dz, double cfl, double *t, double tend, int nsteps,
+                    const orc_scheme *s, double *u) {
+    const size_t N = (size_t)nx * ny * nz;
+    double *work = (double *)malloc(sizeof(double) * 2 * N);
+    int it = 0;
+    while (*t < tend && (nsteps < 0 || it < nsteps)) {
+        double dt = orc_burgers_step(nx, ny, nz, dx, dy, dz, cfl, *t, tend, s, u, work);
+        *t = *t + dt;
+        it++;
+    }
+    free(work);
+    return it;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Heat                                                                      */
+/* ------------------------------------------------------------------------ */
+
+/* Forward Euler with the 7-point operator: Compute_Laplace3d_O2
+ * (D3/kernels.cu:37-64) composed with stage 1 of Compute_RK (:130-132) on the
+ * Dirichlet interior 0<i<n-1; boundary nodes are held. */
+void orc_heat7_fe(unsigned nx, unsigned ny, unsigned nz, double kx, double ky,
+                  double kz, double dt, int nsteps, double *u) {
+    const size_t N = (size_t)nx * ny * nz, xy = (size_t)nx * ny;
+    double *Lu = (double *)calloc(N, sizeof(double));
+    double *uo = (double *)malloc(sizeof(double) * N);
+    for (int it = 0; it < nsteps; it++) {
+        memcpy(uo, u, sizeof(double) * N);
+        orc_lu3d_o2(nx, ny, nz, kx, ky, kz, uo, Lu);
+#pragma omp parallel for collapse(2) schedule(static)
+        for (unsigned k = 1; k < nz - 1; k++)
+            for (unsigned j = 1; j < ny - 1; j++)
+                for (unsigned i = 1; i < nx - 1; i++) {
+                    size_t o = i + (size_t)nx * j + xy * k;
+                    u[o] = rk_node(1, dt, u[o], uo[o], Lu[o]);
+                }
+    }
+    free(Lu);
+    free(uo);
+}
+
+/* MATLAB order-of-accuracy driver, MLD/diffusion3dTest.m:11-76 with
+ * MLD/Laplace3d.m:32-60 ('OMP' strategy): 13-point operator with Dx/12
+ * factors on nodes 3..n-2 (1-based), SSP-RK3, boundary layer reset after each
+ * step, dt clipped after the step as the MATLAB loop does (:65-68). */
+void orc_matlab_heat3d(unsigned nc, double tfinal, double factor, double *l1,
+                       double *l2, double *linf, int *iters) {
+    const unsigned nx = nc, ny = nc, nz = nc;
+    const double D = 1.0, t0 = 0.1, L = 10;
+    const double dx = L / (nx - 1), dy = L / (ny - 1), dz = L / (nz - 1);
+    const double Dx = D / (dx * dx), Dy = D / (dy * dy), Dz = D / (dz * dz);
+    const size_t N = (size_t)nx * ny * nz, xy = (size_t)nx * ny;
+    double *u = (double *)malloc(sizeof(double) * N);
+    double *uo = (double *)malloc(sizeof(double) * N);
+    double *Lu = (double *)calloc(N, sizeof(double));
+    for (unsigned k = 0; k < nz; k++)
+        for (unsigned j = 0; j < ny; j++)
+            for (unsigned i = 0; i < nx; i++) {
+                double x = -L / 2 + i * dx, y = -L / 2 + j * dy, z = -L / 2 + k * dz;
+                u[i + nx * j + xy * k] = exp(-(x * x + y * y + z * z) / (4 * 1.0 * t0));
+            }
+    double dt = 1 / (2 * D * (1 / (dx * dx) + 1 / (dy * dy) + 1 / (dz * dz))) * factor;
+    double t = t0;
+    int it = 0;
+    while (t < tfinal) {
+        memcpy(uo, u, sizeof(double) * N);
+        for (unsigned stage = 1; stage <= 3; stage++) {
+            memset(Lu, 0, sizeof(double) * N);
+            for (unsigned k = 2; k < nz - 2; k++)
+                for (unsigned j = 2; j < ny - 2; j++)
+                    for (unsigned i = 2; i < nx - 2; i++) {
+                        size_t o = i + nx * j + xy * k;
+                        const size_t nx2 = 2 * (size_t)nx, xy2 = 2 * xy;
+                        double c = u[o];
+                        Lu[o] = Dx / 12 * (-u[o - 2] + 16 * u[o - 1] - 30 * c + 16 * u[o + 1] - u[o + 2]) +
+                                Dy / 12 * (-u[o - nx2] + 16 * u[o - nx] - 30 * c + 16 * u[o + nx] - u[o + nx2]) +
+                                Dz / 12 * (-u[o - xy2] + 16 * u[o - xy] - 30 * c + 16 * u[o + xy] - u[o + xy2]);
+                    }
+            for (size_t q = 0; q < N; q++) u[q] = rk_node(stage, dt, u[q], uo[q], Lu[q]);
+        }
+        for (unsigned k = 0; k < nz; k++)
+            for (unsigned j = 0; j < ny; j++)
+                for (unsigned i = 0; i < nx; i++)
+                    if (i == 0 || i == nx - 1 || j == 0 || j == ny - 1 || k == 0 || k == nz - 1)
+                        u[i + nx * j + xy * k] = 0;
+        if (t + dt > tfinal) dt = tfinal - t;
+        it++;
+        t = t + dt;
+    }
+    double s1 = 0, s2 = 0, mx = 0;
+    for (unsigned k = 0; k < nz; k++)
+        for (unsigned j = 0; j < ny; j++)
+            for (unsigned i = 0; i < nx; i++) {
+                double x = -L / 2 + i * dx, y = -L / 2 + j * dy, z = -L / 2 + k * dz;
+                double uE = pow(t0 / tfinal, 1.5) * exp(-(x * x + y * y + z * z) / (4 * 1.0 * tfinal));
+                double e = fabs(uE - u[i + nx * j + xy * k]);
+                s1 += e;
+                s2 += e * e;
+                if (e > mx) mx = e;
+            }
+    *l1 = dx * dy * dz * s1;
+    *l2 = sqrt(dx * dy * dz * s2);
+    *linf = mx;
+    *iters = it;
+    free(u);
+    free(uo);
+    free(Lu);
+}
+
+/* 1-D periodic derivative -(F_{i+1/2}-F_{i-1/2})/dx (order-of-accuracy checks). */
+void orc_rhs_1d_periodic(unsigned n, double dx, const double *u, double *Lu,
+                         const orc_scheme *s) {
+    double *fp = (double *)malloc(sizeof(double) * n), *fm = (double *)malloc(sizeof(double) * n);
+    double *F = (double *)malloc(sizeof(double) * n);
+    for (unsigned i = 0; i < n; i++) orc_split(u[i], s->alpha, s->lf, &fp[i], &fm[i]);
+    for (unsigned i = 0; i < n; i++) {
+        double v[5], w[5];
+        for (int q = 0; q < 5; q++) {
+            v[q] = fp[wrap((long)i - 2 + q, n)];
+            w[q] = fm[wrap((long)i - 1 + q, n)];
+        }
+        F[i] = orc_face(v, w, s);
+    }
+    for (unsigned i = 0; i < n; i++) Lu[i] = -(F[i] - F[wrap((long)i - 1, n)]) / dx;
+    free(fp);
+    free(fm);
+    free(F);
+}

@@ -1,0 +1,132 @@
+"""Fused periodic Burgers stages on the GPU vs the CPU oracle.
+
+Both paths evaluate the same IEEE binary64 expression trees (no contraction),
+so the comparison is bit-for-bit; the north-star tolerance (1e-12 relative
+L-infinity) is asserted as well so a failure reports its size.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from multigpu_advectiondiffusion_amd import solvers
+from multigpu_advectiondiffusion_amd.hip import DeviceArray
+
+pytestmark = pytest.mark.gpu
+
+ORACLE_SCHEME = {"Z2": O.Z2_GLOBAL, "JS": O.JS_GLOBAL, "Z1": dict(weno=O.WENO_Z, p=1, eps=1e-6, lf=O.LF_GLOBAL)}
+TOL = 1e-12
+
+
+def rel_linf(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def oracle_stage(u_in, u_n, stage, dt, d, scheme):
+    alpha = O.maxabs(u_in)
+    Lu = O.rhs_periodic(u_in, d, **ORACLE_SCHEME[scheme], alpha=alpha)
+    out = u_in.copy()
+    O.rk_stage_all(out, u_n, Lu, stage, dt)
+    return out
+
+
+@pytest.mark.parametrize("scheme", ["Z2", "JS", "Z1"])
+@pytest.mark.parametrize("shape", [(32, 32, 32), (48, 32, 24), (16, 48, 20)])
+def test_stages_bitwise(hipdev, scheme, shape):
+    nx, ny, nz = shape
+    s = solvers.PeriodicBurgers(shape, scheme=scheme, cfl=0.4)
+    u0 = solvers.ic_fourier(shape, seed=2)
+    s.set_initial(u0, t0=0.0, tend=10.0)
+    d = (s.dx, s.dy, s.dz)
+    # stage by stage through the C ABI
+    s.stage(1, s.u, s.u, s.w1)
+    st = s.state()
+    dt = 0.4 * s.dx / O.maxabs(u0)
+    assert st.dt == dt
+    u1 = oracle_stage(u0, u0, 1, dt, d, scheme)
+    g1 = s.w1.download()
+    assert rel_linf(g1, u1) <= TOL
+    assert np.array_equal(g1, u1)
+    s.stage(2, s.w1, s.u, s.w2)
+    u2 = oracle_stage(u1, u0, 2, dt, d, scheme)
+    g2 = s.w2.download()
+    assert np.array_equal(g2, u2)
+    s.stage(3, s.w2, s.u, s.u)
+    u3 = oracle_stage(u2, u0, 3, dt, d, scheme)
+    g3 = s.solution()
+    assert np.array_equal(g3, u3)
+    st = s.state()
+    assert st.steps == 1 and st.t == dt
+    assert st.amax == O.maxabs(u3)
+
+
+@pytest.mark.parametrize("scheme", ["Z2", "JS"])
+@pytest.mark.parametrize("graph", [False, True])
+def test_run_to_half_bitwise(hipdev, scheme, graph):
+    """32^3 run to t = 0.5 (past shock formation) against orc_burgers_run."""
+    n = 32
+    s = solvers.PeriodicBurgers(n, scheme=scheme, cfl=0.4)
+    u0 = solvers.ic_fourier(n, seed=2)
+    s.set_initial(u0, t0=0.0, tend=0.5)
+    st = s.run(batch=8, graph=graph)
+    g = s.solution()
+    ref = u0.copy()
+    steps, t = O.burgers_run(ref, 2.0 / n, 0.4, 0.0, 0.5, **ORACLE_SCHEME[scheme])
+    assert st.steps == steps and st.t == t == 0.5
+    assert rel_linf(g, ref) <= TOL
+    assert np.array_equal(g, ref)
+
+
+def test_graph_replays_match_eager(hipdev):
+    """Several graph replays with host synchronisation in between equal eager steps."""
+    n = 32
+    u0 = solvers.ic_fourier(n, seed=7)
+    a = solvers.PeriodicBurgers(n, scheme="Z2")
+    b = solvers.PeriodicBurgers(n, scheme="Z2")
+    a.set_initial(u0, tend=10.0)
+    b.set_initial(u0, tend=10.0)
+    for _ in range(4):
+        a.steps(3, graph=True)
+        a.stream.synchronize()
+        b.steps(3, graph=False)
+        assert np.array_equal(a.solution(), b.solution())
+    assert a.state().steps == 12
+
+
+def test_no_op_after_tend(hipdev):
+    n = 32
+    s = solvers.PeriodicBurgers(n, scheme="JS")
+    s.set_initial(solvers.ic_fourier(n, seed=3), tend=0.05)
+    st = s.run(batch=4)
+    u = s.solution()
+    s.steps(5)
+    st2 = s.state()
+    assert st2.steps == st.steps and st2.t == st.t == 0.05
+    assert np.array_equal(s.solution(), u)
+
+
+def test_256_one_step_vs_oracle(hipdev):
+    """Full-size grid of the WENO5-Z benchmark: one SSP-RK3 step, bitwise."""
+    n = 256
+    s = solvers.PeriodicBurgers(n, scheme="Z2")
+    u0 = solvers.ic_fourier(n, seed=3)
+    s.set_initial(u0, tend=10.0)
+    s.steps(1, graph=False)
+    g = s.solution()
+    ref = u0.copy()
+    O.burgers_step(ref, 2.0 / n, 0.4, 0.0, 10.0, **O.Z2_GLOBAL)
+    assert rel_linf(g, ref) <= TOL
+    assert np.array_equal(g, ref)
+
+
+def test_256_conservation(hipdev):
+    """Periodic conservative scheme: sum(u) is invariant up to rounding over many steps."""
+    n = 256
+    s = solvers.PeriodicBurgers(n, scheme="Z2")
+    u0 = solvers.ic_fourier(n, seed=3)
+    s.set_initial(u0, tend=10.0)
+    s.steps(20)
+    g = s.solution()
+    assert np.isfinite(g).all()
+    m0, m1 = float(np.sum(u0)), float(np.sum(g))
+    assert abs(m1 - m0) <= 1e-11 * np.sum(np.abs(u0))
+    assert np.max(np.abs(g)) <= np.max(np.abs(u0)) * (1 + 1e-6)
