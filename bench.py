@@ -76,6 +76,8 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="after timing, run 3 device copies of one field (counter calibration)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -145,6 +147,12 @@ def main() -> int:
         },
         "cpu_baseline": None,
     }
+    if args.copy_calib:
+        # plain device-to-device copies of one field (n^3 * 8 bytes each way),
+        # the known-size reference for scaling FETCH_SIZE / WRITE_SIZE
+        for _ in range(3):
+            s.w1.copy_from(s.u, s.stream)
+        s.stream.synchronize()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(n, args.scheme, args.seed)
     print(json.dumps(out), flush=True)

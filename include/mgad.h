@@ -172,6 +172,91 @@ int mgad_burgers_steps(hipStream_t stream, const mgad_scheme *scheme,
                        double dx, double dy, double dz, double *u, double *w1,
                        double *w2, void *ctrl, int nsteps, int use_graph);
 
+/* Storage precision of the fused path's fields. */
+#define MGAD_F64 0  /* double storage, double arithmetic */
+#define MGAD_F32 1  /* float storage; split fluxes, smoothness indicators,
+                       weights and the RK combine in double, one rounding to
+                       float per stored value */
+
+/* mgad_burgers_init / mgad_burgers_steps with the storage precision chosen
+ * by `precision`; u, w1, w2 are float* for MGAD_F32, double* for MGAD_F64. */
+int mgad_burgers_init_p(int precision, hipStream_t stream, unsigned int nx,
+                        unsigned int ny, unsigned int nz, const void *u,
+                        void *ctrl, double t0, double tend, double cfl);
+int mgad_burgers_steps_p(int precision, hipStream_t stream,
+                         const mgad_scheme *scheme, unsigned int nx,
+                         unsigned int ny, unsigned int nz, double dx,
+                         double dy, double dz, void *u, void *w1, void *w2,
+                         void *ctrl, int nsteps, int use_graph);
+
+/* Autotuner: times stage-1 launches (u -> w1) of every candidate tile width
+ * (16, 32) and z-chunk depth (4..128 planes, nz) on the caller's buffers and
+ * records the fastest for this (precision, scheme, grid); later launches of
+ * that grid use it.  Overwrites w1 and ctrl: call mgad_burgers_init[_p]
+ * afterwards.  Results do not depend on the configuration (bit-identical). */
+int mgad_burgers_autotune(hipStream_t stream, int precision,
+                          const mgad_scheme *scheme, unsigned int nx,
+                          unsigned int ny, unsigned int nz, double dx,
+                          double dy, double dz, const void *u, void *w1,
+                          void *ctrl, int *tile, int *kchunk);
+/* Sets the launch configuration of a grid explicitly (tile 0 / kchunk 0 =
+ * the built-in default). */
+int mgad_burgers_set_launch(int precision, const mgad_scheme *scheme,
+                            unsigned int nx, unsigned int ny, unsigned int nz,
+                            int tile, int kchunk);
+
+/* ------------------------------------------------------------------------ */
+/* Batched periodic Burgers: many independent problems, one launch per stage */
+/* ------------------------------------------------------------------------ */
+
+/* One problem of a batch: its three fields (precision of the batch), its own
+ * MGAD_CTRL_BYTES control buffer, extents (multiples of 16 in x and y) and
+ * spacings. */
+typedef struct {
+    void *u, *w1, *w2;
+    void *ctrl;
+    unsigned int nx, ny, nz;
+    double dx, dy, dz;
+} mgad_problem;
+
+/* Filled by mgad_batch_init; passed to the step calls. */
+typedef struct {
+    void *workspace;        /* device buffer holding descriptors + work list */
+    size_t workspace_bytes;
+    int nprob, nitems;      /* problems, (problem, tile) work items */
+    int grid;               /* persistent blocks per stage launch */
+    int kchunk;             /* planes per work item (autotuned if 0 given) */
+    int precision;          /* MGAD_F64 or MGAD_F32 */
+    mgad_scheme scheme;
+} mgad_batch;
+
+/* Device bytes the workspace of this batch needs (upper bound over the
+ * chunk depths the autotuner tries). */
+size_t mgad_batch_workspace_bytes(int nprob, const mgad_problem *probs);
+
+/* Builds the work list in the caller's workspace and initialises every
+ * problem's ctrl at t0 (u of each problem must hold its initial field).
+ * kchunk 0 autotunes the planes per work item by timing the batched stage.
+ * Synchronises the stream. */
+int mgad_batch_init(hipStream_t stream, int precision,
+                    const mgad_scheme *scheme, int nprob,
+                    const mgad_problem *probs, void *workspace,
+                    size_t workspace_bytes, int kchunk, double t0,
+                    double tend, double cfl, mgad_batch *out);
+
+/* nsteps SSP-RK3 steps of every problem: three persistent launches per step
+ * (each walks the whole work list); use_graph captures one step in a
+ * hipGraph and replays it.  Each problem keeps its own dt and LF alpha, and
+ * its results are bit-identical to advancing it alone. */
+int mgad_batch_steps(hipStream_t stream, const mgad_batch *batch, int nsteps,
+                     int use_graph);
+
+/* The same steps with the single-problem kernel launched problem by problem
+ * (3 launches per problem per step, each with its tuned configuration). */
+int mgad_batch_steps_serial(hipStream_t stream, const mgad_batch *batch,
+                            const mgad_problem *probs, int nsteps,
+                            int use_graph);
+
 /* Copies ctrl to the host (synchronises the stream). */
 int mgad_burgers_read_ctrl(hipStream_t stream, const void *ctrl,
                            mgad_burgers_state *out);

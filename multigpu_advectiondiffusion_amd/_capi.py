@@ -15,6 +15,7 @@ MGAD_OK, MGAD_ERR_INVALID_ARG, MGAD_ERR_UNSUPPORTED, MGAD_ERR_HIP, MGAD_ERR_NONF
 WENO_Z, WENO_JS = 0, 1
 LF_LOCAL, LF_GLOBAL = 0, 1
 CTRL_BYTES = 256
+F64, F32 = 0, 1
 
 # Every symbol include/mgad.h declares.
 EXPORTS = (
@@ -23,6 +24,8 @@ EXPORTS = (
     "Call_sspRK", "Call_Lu3d", "Call_RK3d",
     "mgad_set_adv_scheme", "mgad_lu3d_o2",
     "mgad_burgers_init", "mgad_burgers_stage", "mgad_burgers_steps", "mgad_burgers_read_ctrl",
+    "mgad_burgers_init_p", "mgad_burgers_steps_p", "mgad_burgers_autotune", "mgad_burgers_set_launch",
+    "mgad_batch_workspace_bytes", "mgad_batch_init", "mgad_batch_steps", "mgad_batch_steps_serial",
     "mgad_heat7_fe_steps", "mgad_ic_fourier", "mgad_ic_gauss_noise",
 )
 
@@ -41,6 +44,20 @@ class BurgersState(ctypes.Structure):
     """mgad_burgers_state."""
     _fields_ = [("t", ctypes.c_double), ("dt", ctypes.c_double), ("tend", ctypes.c_double),
                 ("cfl", ctypes.c_double), ("amax", ctypes.c_double), ("steps", ctypes.c_uint64)]
+
+
+class Problem(ctypes.Structure):
+    """mgad_problem."""
+    _fields_ = [("u", ctypes.c_void_p), ("w1", ctypes.c_void_p), ("w2", ctypes.c_void_p), ("ctrl", ctypes.c_void_p),
+                ("nx", ctypes.c_uint), ("ny", ctypes.c_uint), ("nz", ctypes.c_uint),
+                ("dx", ctypes.c_double), ("dy", ctypes.c_double), ("dz", ctypes.c_double)]
+
+
+class Batch(ctypes.Structure):
+    """mgad_batch."""
+    _fields_ = [("workspace", ctypes.c_void_p), ("workspace_bytes", ctypes.c_size_t), ("nprob", ctypes.c_int),
+                ("nitems", ctypes.c_int), ("grid", ctypes.c_int), ("kchunk", ctypes.c_int),
+                ("precision", ctypes.c_int), ("scheme", Scheme)]
 
 
 class MgadError(RuntimeError):
@@ -86,6 +103,24 @@ def lib():
         L.mgad_burgers_stage.restype = I
         L.mgad_burgers_steps.argtypes = [VP, ctypes.POINTER(Scheme), U, U, U, D, D, D, VP, VP, VP, VP, I, I]
         L.mgad_burgers_steps.restype = I
+        L.mgad_burgers_init_p.argtypes = [I, VP, U, U, U, VP, VP, D, D, D]
+        L.mgad_burgers_init_p.restype = I
+        L.mgad_burgers_steps_p.argtypes = [I, VP, ctypes.POINTER(Scheme), U, U, U, D, D, D, VP, VP, VP, VP, I, I]
+        L.mgad_burgers_steps_p.restype = I
+        IP = ctypes.POINTER(ctypes.c_int)
+        L.mgad_burgers_autotune.argtypes = [VP, I, ctypes.POINTER(Scheme), U, U, U, D, D, D, VP, VP, VP, IP, IP]
+        L.mgad_burgers_autotune.restype = I
+        L.mgad_burgers_set_launch.argtypes = [I, ctypes.POINTER(Scheme), U, U, U, I, I]
+        L.mgad_burgers_set_launch.restype = I
+        PP, BP = ctypes.POINTER(Problem), ctypes.POINTER(Batch)
+        L.mgad_batch_workspace_bytes.argtypes = [I, PP]
+        L.mgad_batch_workspace_bytes.restype = ctypes.c_size_t
+        L.mgad_batch_init.argtypes = [VP, I, ctypes.POINTER(Scheme), I, PP, VP, ctypes.c_size_t, I, D, D, D, BP]
+        L.mgad_batch_init.restype = I
+        L.mgad_batch_steps.argtypes = [VP, BP, I, I]
+        L.mgad_batch_steps.restype = I
+        L.mgad_batch_steps_serial.argtypes = [VP, BP, PP, I, I]
+        L.mgad_batch_steps_serial.restype = I
         L.mgad_burgers_read_ctrl.argtypes = [VP, VP, ctypes.POINTER(BurgersState)]
         L.mgad_burgers_read_ctrl.restype = I
         L.mgad_heat7_fe_steps.argtypes = [VP, U, U, U, D, D, D, D, VP, VP, I, I]

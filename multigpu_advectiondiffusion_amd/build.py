@@ -18,7 +18,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libmgad.so")
 ARCH = os.environ.get("MGAD_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["burgers_periodic.hip", "dropin.hip", "heat.hip", "host.cpp"]
+SOURCES = ["burgers_periodic.hip", "batch.hip", "dropin.hip", "heat.hip", "host.cpp"]
 
 HIPCC_FLAGS = [
     "-O3",

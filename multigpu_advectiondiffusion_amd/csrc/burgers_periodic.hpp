@@ -1,0 +1,281 @@
+// burgers_periodic.hpp -- shared declarations of the fused periodic Burgers
+// stage (single-problem launches and the batched persistent kernel).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "mgad_common.hpp"
+
+namespace mgad {
+
+// Layout of the caller's MGAD_CTRL_BYTES device buffer.
+struct Ctrl {
+    double t;                    // time at the start of the next step
+    double dt;                   // dt of the current step, 0 once done
+    double tend;
+    double cfl;
+    unsigned long long amax[3];  // max|u| of: [0] u^n, [1] u^(1), [2] u^(2)
+    unsigned long long steps;
+};
+static_assert(sizeof(Ctrl) <= MGAD_CTRL_BYTES, "ctrl layout");
+
+// Grid of one problem: extents, spacings and their correctly rounded
+// reciprocals (for div_by), planes per z chunk.
+struct GridArgs {
+    int nx, ny, nz, kchunk;
+    double dx, dy, dz, rdx, rdy, rdz;
+};
+
+__device__ __forceinline__ int wrapi(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
+
+// Maps the launch's linear block id to a tile so that the blocks one XCD
+// runs (ids congruent mod 8) are contiguous tiles sharing halos in L2.
+__device__ __forceinline__ unsigned xcd_tile(unsigned bid, unsigned nblocks) {
+    if ((nblocks & 7u) != 0u) return bid;
+    return (bid & 7u) * (nblocks >> 3) + (bid >> 3);
+}
+
+// Step control shared by every block of a stage.  Stage 1 derives dt from
+// max|u^n| (all blocks compute the same value) and records it; stage 2 moves
+// t forward.  Each stage's lead thread clears the max|u| slot the next-but-one
+// stage accumulates into.  Returns false once the run has reached tend.
+__device__ __forceinline__ bool stage_control(Ctrl *ctrl, int stage, double dx, bool lead, double &dt,
+                                              double &alpha) {
+    if (stage == 1) {
+        const double t = ctrl->t, tend = ctrl->tend;
+        if (!(t < tend)) {
+            if (lead) ctrl->dt = 0.0;
+            return false;
+        }
+        const double a = bits_double(ctrl->amax[0]);
+        dt = ctrl->cfl * dx / a;
+        if (t + dt > tend) dt = tend - t;
+        alpha = a;
+        if (lead) {
+            ctrl->dt = dt;
+            ctrl->amax[2] = 0ull;
+        }
+        return true;
+    }
+    dt = ctrl->dt;
+    if (dt == 0.0) return false;
+    alpha = bits_double(ctrl->amax[stage - 1]);
+    if (lead) {
+        if (stage == 2) {
+            ctrl->amax[0] = 0ull;
+            ctrl->t = ctrl->t + dt;
+            ctrl->steps = ctrl->steps + 1ull;
+        } else {
+            ctrl->amax[1] = 0ull;
+        }
+    }
+    return true;
+}
+
+template <class T>
+__device__ __forceinline__ double ld(const T *p, unsigned o) {
+    return static_cast<double>(p[o]);
+}
+
+// LDS of one tile.
+template <int TX, int TY>
+struct TileSmem {
+    double2 P[TY + 6][TX + 6];  // (f+, f-) of the current plane, tile + halo
+    double FX[TY][TX + 1];      // x faces x0-1/2 .. x0+TX-1/2
+    double FY[TY + 1][TX];      // y faces y0-1/2 .. y0+TY-1/2
+    unsigned long long red[TX * TY / 64];
+};
+
+// One tile x z-chunk of one stage; ends with one atomicMax per block into the
+// stage's max|u| slot.
+template <class T, int WENO, int P, int TX, int TY>
+__device__ __forceinline__ void stage_tile(const T *__restrict__ uin, const T *un, T *uout,
+                                           const GridArgs &g, int stage, unsigned tile, double dt,
+                                           double alpha, double eps, TileSmem<TX, TY> &sm,
+                                           unsigned long long *amax_slot) {
+    constexpr int NT = TX * TY;
+    constexpr int NW = NT / 64;
+    static_assert(NT % 64 == 0, "whole waves");
+    static_assert(6 * TX + 6 * TY <= NT, "one halo node per thread at most");
+    static_assert(TX + TY <= 64, "edge faces fit in one wave");
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const int tid = threadIdx.x;
+    const int ntx = nx / TX, nty = ny / TY;
+    const int bx = tile % ntx;
+    const int by = (tile / ntx) % nty;
+    const int bz = tile / (ntx * nty);
+    const int x0 = bx * TX, y0 = by * TY;
+    const int z0 = bz * g.kchunk;
+    const int z1 = min(z0 + g.kchunk, nz);
+    const int tx = tid % TX, ty = tid / TX;
+    const int wave = tid >> 6, lane = tid & 63;
+    const unsigned plane = static_cast<unsigned>(nx) * static_cast<unsigned>(ny);
+    const unsigned col = static_cast<unsigned>(x0 + tx) + static_cast<unsigned>(nx) * (y0 + ty);
+
+    // Halo node handled by this thread: 6*TY x-halo nodes, then 6*TX y-halo nodes.
+    int hl_r = 0, hl_c = 0;
+    unsigned hcol = 0;
+    const bool has_halo = tid < 6 * TY + 6 * TX;
+    if (tid < 6 * TY) {
+        const int r = tid / 6, c = tid % 6;
+        hl_r = r + 3;
+        hl_c = c < 3 ? c : TX + c;
+        hcol = static_cast<unsigned>(wrapi(x0 + hl_c - 3, nx)) + static_cast<unsigned>(nx) * (y0 + r);
+    } else if (has_halo) {
+        const int h = tid - 6 * TY;
+        const int c = h % TX, q = h / TX;
+        hl_r = q < 3 ? q : TY + q;
+        hl_c = c + 3;
+        hcol = static_cast<unsigned>(x0 + c) + static_cast<unsigned>(nx) * wrapi(y0 + hl_r - 3, ny);
+    }
+
+    // z register queues: qp/qm[q] hold the split fluxes of plane k-3+q,
+    // sp/sn[q] the 13/12 terms centred on plane k-1+q, qu[q] u_in of plane k+q.
+    double qp[7], qm[7], sp[4], sn[4], qu[4];
+#pragma unroll
+    for (int q = 0; q < 6; q++) {
+        const double v = ld(uin, col + plane * static_cast<unsigned>(wrapi(z0 - 3 + q, nz)));
+        lf_split(v, alpha, qp[q], qm[q]);
+        if (q >= 3) qu[q - 3] = v;
+    }
+    // 13/12 terms at centres z0-2 .. z0+1 (index 0..3 here, shifted below)
+    const double sp0 = sterm(qp[0], qp[1], qp[2]);
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+        sp[q] = sterm(qp[q + 1], qp[q + 2], qp[q + 3]);
+        sn[q] = sterm(qm[q + 1], qm[q + 2], qm[q + 3]);
+    }
+    // face z0-1/2: f+ at z0-3..z0+1 (centres z0-2..z0), f- at z0-2..z0+2 (centres z0-1..z0+1)
+    double fz_prev = weno_face_s<WENO, P>(qp[0], qp[1], qp[2], qp[3], qp[4], qm[1], qm[2], qm[3],
+                                          qm[4], qm[5], sp0, sp[0], sp[1], sn[0], sn[1], sn[2], eps);
+
+    const T *own_ptr = uin + col;
+    const T *un_ptr = un + col;
+    T *out_ptr = uout + col;
+    const T *halo_ptr = uin + hcol;
+    int kn = z0 + 3 >= nz ? z0 + 3 - nz : z0 + 3;  // plane of the next own-column load
+    double own_next = ld(own_ptr, plane * static_cast<unsigned>(kn));
+    double halo_next = has_halo ? ld(halo_ptr, plane * static_cast<unsigned>(z0)) : 0.0;
+    double un_next = (stage == 1) ? 0.0 : ld(un_ptr, plane * static_cast<unsigned>(z0));
+    unsigned long long vmax = 0ull;
+
+    for (int k = z0; k < z1; k++) {
+        const double own = own_next, halo = halo_next, unv = un_next;
+        if (k + 1 < z1) {
+            kn = kn + 1 == nz ? 0 : kn + 1;
+            own_next = ld(own_ptr, plane * static_cast<unsigned>(kn));
+            if (has_halo) halo_next = ld(halo_ptr, plane * static_cast<unsigned>(k + 1));
+            if (stage != 1) un_next = ld(un_ptr, plane * static_cast<unsigned>(k + 1));
+        }
+        lf_split(own, alpha, qp[6], qm[6]);
+        qu[3] = own;
+        sp[3] = sterm(qp[4], qp[5], qp[6]);  // centre k+2
+        sn[3] = sterm(qm[4], qm[5], qm[6]);
+        sm.P[ty + 3][tx + 3] = make_double2(qp[3], qm[3]);
+        if (has_halo) {
+            double hp, hm;
+            lf_split(halo, alpha, hp, hm);
+            sm.P[hl_r][hl_c] = make_double2(hp, hm);
+        }
+        __syncthreads();
+
+        // z face k+1/2 from the register queue; x face x+1/2 and y face y+1/2 from LDS
+        const double fz = weno_face_s<WENO, P>(qp[1], qp[2], qp[3], qp[4], qp[5], qm[2], qm[3], qm[4],
+                                               qm[5], qm[6], sp[0], sp[1], sp[2], sn[1], sn[2], sn[3],
+                                               eps);
+        {
+            const double2 a0 = sm.P[ty + 3][tx + 1], a1 = sm.P[ty + 3][tx + 2], a2 = sm.P[ty + 3][tx + 3],
+                          a3 = sm.P[ty + 3][tx + 4], a4 = sm.P[ty + 3][tx + 5], a5 = sm.P[ty + 3][tx + 6];
+            sm.FX[ty][tx + 1] = weno_face2<WENO, P>(a0.x, a1.x, a2.x, a3.x, a4.x, a1.y, a2.y, a3.y,
+                                                    a4.y, a5.y, eps);
+        }
+        {
+            const double2 a0 = sm.P[ty + 1][tx + 3], a1 = sm.P[ty + 2][tx + 3], a2 = sm.P[ty + 3][tx + 3],
+                          a3 = sm.P[ty + 4][tx + 3], a4 = sm.P[ty + 5][tx + 3], a5 = sm.P[ty + 6][tx + 3];
+            sm.FY[ty + 1][tx] = weno_face2<WENO, P>(a0.x, a1.x, a2.x, a3.x, a4.x, a1.y, a2.y, a3.y,
+                                                    a4.y, a5.y, eps);
+        }
+        // faces on the tile's low edges, by one wave that rotates with k
+        if (wave == k % NW && lane < TY + TX) {
+            double2 a0, a1, a2, a3, a4, a5;
+            if (lane < TY) {
+                const int r = lane + 3;
+                a0 = sm.P[r][0]; a1 = sm.P[r][1]; a2 = sm.P[r][2];
+                a3 = sm.P[r][3]; a4 = sm.P[r][4]; a5 = sm.P[r][5];
+            } else {
+                const int c = lane - TY + 3;
+                a0 = sm.P[0][c]; a1 = sm.P[1][c]; a2 = sm.P[2][c];
+                a3 = sm.P[3][c]; a4 = sm.P[4][c]; a5 = sm.P[5][c];
+            }
+            const double f = weno_face2<WENO, P>(a0.x, a1.x, a2.x, a3.x, a4.x, a1.y, a2.y, a3.y, a4.y,
+                                                 a5.y, eps);
+            if (lane < TY)
+                sm.FX[lane][0] = f;
+            else
+                sm.FY[0][lane - TY] = f;
+        }
+        __syncthreads();
+
+        double Lu = -div_by(sm.FX[ty][tx + 1] - sm.FX[ty][tx], g.dx, g.rdx);
+        Lu -= div_by(sm.FY[ty + 1][tx] - sm.FY[ty][tx], g.dy, g.rdy);
+        Lu -= div_by(fz - fz_prev, g.dz, g.rdz);
+        const double uo = (stage == 1) ? qu[0] : unv;
+        double unew;
+        if (stage == 1)
+            unew = uo + dt * Lu;
+        else if (stage == 2)
+            unew = 0.75 * uo + 0.25 * (qu[0] + dt * Lu);
+        else
+            unew = div_by(uo + 2.0 * (qu[0] + dt * Lu), 3.0, 1.0 / 3.0);
+        const T stored = static_cast<T>(unew);
+        out_ptr[plane * static_cast<unsigned>(k)] = stored;
+        const unsigned long long b = abs_bits(static_cast<double>(stored));
+        vmax = b > vmax ? b : vmax;
+
+        fz_prev = fz;
+#pragma unroll
+        for (int q = 0; q < 6; q++) {
+            qp[q] = qp[q + 1];
+            qm[q] = qm[q + 1];
+        }
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            sp[q] = sp[q + 1];
+            sn[q] = sn[q + 1];
+            qu[q] = qu[q + 1];
+        }
+    }
+
+    // ---- block max -> one atomic per block ----
+    vmax = wave_max_u64(vmax);
+    if (lane == 0) sm.red[wave] = vmax;
+    __syncthreads();
+    if (tid == 0) {
+        unsigned long long m = sm.red[0];
+#pragma unroll
+        for (int w = 1; w < NW; w++) m = sm.red[w] > m ? sm.red[w] : m;
+        atomicMax(amax_slot, m);
+    }
+}
+
+int default_kchunk(int nx, int ny, int nz, int tx, int ty);
+
+// Tile width (16 or 32; 0 = widest that divides nx) and planes per block
+// (0 = default_kchunk) of the single-problem stage launch.
+struct LaunchCfg {
+    int tile, kchunk;
+};
+LaunchCfg launch_cfg(int precision, const mgad_scheme &sc, unsigned nx, unsigned ny, unsigned nz);
+void set_launch_cfg(int precision, const mgad_scheme &sc, unsigned nx, unsigned ny, unsigned nz, LaunchCfg c);
+bool valid_grid(unsigned nx, unsigned ny, unsigned nz);
+GridArgs make_grid(unsigned nx, unsigned ny, unsigned nz, double dx, double dy, double dz, int kchunk);
+
+template <class T>
+int burgers_stage_launch_t(hipStream_t stream, const mgad_scheme &sc, unsigned nx, unsigned ny,
+                           unsigned nz, double dx, double dy, double dz, int stage, const T *u_in,
+                           const T *u_n, T *u_out, void *ctrl, int kchunk, int tile_pref);
+template <class T>
+int burgers_init_t(hipStream_t stream, unsigned nx, unsigned ny, unsigned nz, const T *u, void *ctrl,
+                   double t0, double tend, double cfl);
+
+}  // namespace mgad

@@ -36,6 +36,26 @@ constexpr double kC1312 = 13. / 12., kC14 = 1. / 4.;
 
 enum : int { kWenoZ = 0, kWenoJS = 1 };
 
+// Correctly rounded a/b for a divisor fixed on the host, given rb = RN(1/b)
+// computed there: q = RN(a*rb), rem = a - b*q (exact with an FMA), and
+// RN(q + rem*rb) = RN(a/b) (Markstein's theorem; no over/underflow in the
+// range taken).  Operands below 2^-900 in magnitude, zeros included, take the
+// IEEE division so signs of zero and subnormal quotients are unchanged.
+__device__ __forceinline__ double div_by(double a, double b, double rb) {
+    if (__builtin_expect(fabs(a) < 0x1p-900, 0)) return a / b;
+    const double q = a * rb;
+    const double rem = __builtin_fma(-b, q, a);
+    return __builtin_fma(rem, rb, q);
+}
+
+// First term of a smoothness indicator, 13/12 (a-2b+c)^2 (kernels.cu:157-159):
+// it depends only on the sub-stencil's centre, so callers that march along a
+// line compute it once per node and pass it to weno_side_s.
+__device__ __forceinline__ double sterm(double a, double b, double c) {
+    const double s = (a - 2.0 * b) + c;
+    return (kC1312 * s) * s;
+}
+
 // Jiang-Shu smoothness indicators of the three 3-point sub-stencils
 // (kernels.cu:157-159): 13/12 (a-2b+c)^2 + 1/4 (a-4b+3c)^2, etc.
 __device__ __forceinline__ double smooth0(double a, double b, double c) {
@@ -62,9 +82,74 @@ __device__ __forceinline__ double zweight(double d, double tau5, double beta, do
     return d * (1.0 + (P == 2 ? q * q : q));
 }
 
+// One side of the reconstruction with the 13/12 terms S0..S2 supplied:
+// PLUS: left-biased f+ side, v = f+ at i-2..i+2, weights (1,6,3)/10;
+// else the right-biased f- side, v = f- at i-1..i+3, weights (3,6,1)/10.
+// B_k = S_k + 1/4 q_k^2 is the reference's sum in its own order.
+template <int WENO, int P, bool PLUS>
+__device__ __forceinline__ double weno_side_s(double v0, double v1, double v2, double v3, double v4,
+                                              double S0, double S1, double S2, double eps) {
+    const double q0 = (v0 - 4.0 * v1) + 3.0 * v2;
+    const double q1 = v1 - v3;
+    const double q2 = (3.0 * v2 - 4.0 * v3) + v4;
+    const double B0 = S0 + (kC14 * q0) * q0;
+    const double B1 = S1 + (kC14 * q1) * q1;
+    const double B2 = S2 + (kC14 * q2) * q2;
+    double c0, c1, c2, d0, d1, d2;
+    if (PLUS) {
+        c0 = (2.0 * v0 - 7.0 * v1) + 11.0 * v2;
+        c1 = (-v1 + 5.0 * v2) + 2.0 * v3;
+        c2 = (2.0 * v2 + 5.0 * v3) - v4;
+        d0 = kD0N; d1 = kD1N; d2 = kD2N;
+    } else {
+        c0 = (-v0 + 5.0 * v1) + 2.0 * v2;
+        c1 = (2.0 * v1 + 5.0 * v2) - v3;
+        c2 = (11.0 * v2 - 7.0 * v3) + 2.0 * v4;
+        d0 = kD0P; d1 = kD1P; d2 = kD2P;
+    }
+    if (WENO == kWenoZ) {
+        const double tau5 = fabs(B0 - B2);
+        const double a0 = zweight<P>(d0, tau5, B0, eps);
+        const double a1 = zweight<P>(d1, tau5, B1, eps);
+        const double a2 = zweight<P>(d2, tau5, B2, eps);
+        const double sum = (a0 + a1) + a2;
+        return ((a0 * c0 + a1 * c1) + a2 * c2) / (6.0 * sum);
+    } else {
+        const double a0 = d0 / ((eps + B0) * (eps + B0));
+        const double a1 = d1 / ((eps + B1) * (eps + B1));
+        const double a2 = d2 / ((eps + B2) * (eps + B2));
+        const double sum = (a0 + a1) + a2;
+        const double o0 = a0 / sum, o1 = a1 / sum, o2 = a2 / sum;
+        return ((o0 * c0 + o1 * c1) + o2 * c2) / 6.0;
+    }
+}
+
+// Face flux from split fluxes with the 13/12 terms precomputed:
+// Sp_k at the centres of the f+ sub-stencils (i-1, i, i+1), Sm_k at the
+// centres of the f- sub-stencils (i, i+1, i+2).
+template <int WENO, int P>
+__device__ __forceinline__ double weno_face_s(double v0, double v1, double v2, double v3, double v4,
+                                              double w0, double w1, double w2, double w3, double w4,
+                                              double Sp0, double Sp1, double Sp2, double Sm0,
+                                              double Sm1, double Sm2, double eps) {
+    const double hp = weno_side_s<WENO, P, true>(v0, v1, v2, v3, v4, Sp0, Sp1, Sp2, eps);
+    const double hm = weno_side_s<WENO, P, false>(w0, w1, w2, w3, w4, Sm0, Sm1, Sm2, eps);
+    return hp + hm;
+}
+
 // Reconstructed flux at face i+1/2:  v = f+ at i-2..i+2, w = f- at i-1..i+3.
 // WENO-Z: kernels.cu:142-192 (alpha / (6*sum)).
 // WENO-JS: MultiGPU/Burgers3d_Baseline/Kernels.cu:112-220 (w = alpha/sum, /6).
+template <int WENO, int P>
+__device__ __forceinline__ double weno_face2(double v0, double v1, double v2, double v3, double v4,
+                                             double w0, double w1, double w2, double w3, double w4,
+                                             double eps) {
+    return weno_face_s<WENO, P>(v0, v1, v2, v3, v4, w0, w1, w2, w3, w4, sterm(v0, v1, v2),
+                                sterm(v1, v2, v3), sterm(v2, v3, v4), sterm(w0, w1, w2),
+                                sterm(w1, w2, w3), sterm(w2, w3, w4), eps);
+}
+
+// Straight restatement of the same face (kept for the drop-in sweeps).
 template <int WENO, int P>
 __device__ __forceinline__ double weno_face(double v0, double v1, double v2, double v3, double v4,
                                             double w0, double w1, double w2, double w3, double w4,

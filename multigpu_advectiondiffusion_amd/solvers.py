@@ -61,7 +61,7 @@ class PeriodicBurgers:
     LF + SSP-RK3, dt = cfl*dx/max|u^n| clipped to tend (MLB/LFWENO5FDM3d.m:71)."""
 
     def __init__(self, shape, scheme: str = "Z2", cfl: float = 0.4, lo: float = -1.0, hi: float = 1.0,
-                 stream: Stream | None = None):
+                 stream: Stream | None = None, precision: str = "f64"):
         self.nx, self.ny, self.nz = (shape, shape, shape) if np.isscalar(shape) else shape
         self.dx = (hi - lo) / self.nx
         self.dy = (hi - lo) / self.ny
@@ -69,10 +69,12 @@ class PeriodicBurgers:
         self.cfl = cfl
         self.scheme = scheme_struct(scheme)
         self.stream = stream if stream is not None else Stream()
+        self.precision = {"f64": _capi.F64, "f32": _capi.F32}[precision]
+        dtype = np.float64 if self.precision == _capi.F64 else np.float32
         shp = (self.nz, self.ny, self.nx)
-        self.u = DeviceArray(shp)
-        self.w1 = DeviceArray(shp)
-        self.w2 = DeviceArray(shp)
+        self.u = DeviceArray(shp, dtype)
+        self.w1 = DeviceArray(shp, dtype)
+        self.w2 = DeviceArray(shp, dtype)
         self.ctrl = DeviceArray((_capi.CTRL_BYTES,), np.uint8)
 
     @property
@@ -80,9 +82,10 @@ class PeriodicBurgers:
         return self.nx * self.ny * self.nz
 
     def set_initial(self, u0: np.ndarray, t0: float = 0.0, tend: float = float("inf")) -> None:
+        """Upload u0 (rounded once to float for FP32 storage) and start a run at t0."""
         self.u.upload(u0)
-        rc = _capi.lib().mgad_burgers_init(self.stream.handle, self.nx, self.ny, self.nz, self.u.ptr,
-                                           self.ctrl.ptr, t0, tend, self.cfl)
+        rc = _capi.lib().mgad_burgers_init_p(self.precision, self.stream.handle, self.nx, self.ny, self.nz,
+                                             self.u.ptr, self.ctrl.ptr, t0, tend, self.cfl)
         _capi.check(rc, "mgad_burgers_init")
 
     def stage(self, stage: int, u_in: DeviceArray, u_n: DeviceArray, u_out: DeviceArray) -> None:
@@ -92,10 +95,10 @@ class PeriodicBurgers:
         _capi.check(rc, "mgad_burgers_stage")
 
     def steps(self, nsteps: int, graph: bool = True) -> None:
-        rc = _capi.lib().mgad_burgers_steps(self.stream.handle, ctypes.byref(self.scheme), self.nx, self.ny,
-                                            self.nz, self.dx, self.dy, self.dz, self.u.ptr, self.w1.ptr,
-                                            self.w2.ptr, self.ctrl.ptr, nsteps, 1 if graph else 0)
-        _capi.check(rc, "mgad_burgers_steps")
+        rc = _capi.lib().mgad_burgers_steps_p(self.precision, self.stream.handle, ctypes.byref(self.scheme),
+                                              self.nx, self.ny, self.nz, self.dx, self.dy, self.dz, self.u.ptr,
+                                              self.w1.ptr, self.w2.ptr, self.ctrl.ptr, nsteps, 1 if graph else 0)
+        _capi.check(rc, "mgad_burgers_steps_p")
 
     def state(self) -> BurgersState:
         s = _capi.BurgersState()
@@ -116,18 +119,107 @@ class PeriodicBurgers:
         return self.u.download()
 
 
+def autotune(solver: "PeriodicBurgers") -> tuple:
+    """Pick the fastest tile width / z-chunk depth for this solver's grid
+    (times stage launches on its buffers); re-initialise afterwards."""
+    t, k = ctypes.c_int(), ctypes.c_int()
+    rc = _capi.lib().mgad_burgers_autotune(solver.stream.handle, solver.precision, ctypes.byref(solver.scheme),
+                                           solver.nx, solver.ny, solver.nz, solver.dx, solver.dy, solver.dz,
+                                           solver.u.ptr, solver.w1.ptr, solver.ctrl.ptr, ctypes.byref(t),
+                                           ctypes.byref(k))
+    _capi.check(rc, "mgad_burgers_autotune")
+    return t.value, k.value
+
+
+def set_launch(scheme: str, shape, tile: int = 0, kchunk: int = 0, precision: str = "f64") -> None:
+    nx, ny, nz = (shape, shape, shape) if np.isscalar(shape) else shape
+    prec = {"f64": _capi.F64, "f32": _capi.F32}[precision]
+    rc = _capi.lib().mgad_burgers_set_launch(prec, ctypes.byref(scheme_struct(scheme)), nx, ny, nz, tile, kchunk)
+    _capi.check(rc, "mgad_burgers_set_launch")
+
+
+class BurgersBatch:
+    """Independent periodic Burgers problems of uneven cubic sizes advanced
+    together (one persistent launch per SSP-RK3 stage), each with its own dt."""
+
+    def __init__(self, sizes, scheme: str = "Z2", cfl: float = 0.4, precision: str = "f64", lo: float = -1.0,
+                 hi: float = 1.0, stream: Stream | None = None):
+        self.sizes = [tuple(s) if not np.isscalar(s) else (s, s, s) for s in sizes]
+        self.scheme = scheme_struct(scheme)
+        self.cfl = cfl
+        self.precision = {"f64": _capi.F64, "f32": _capi.F32}[precision]
+        self.dtype = np.float64 if self.precision == _capi.F64 else np.float32
+        self.stream = stream if stream is not None else Stream()
+        self.fields = []
+        probs = []
+        for (nx, ny, nz) in self.sizes:
+            shp = (nz, ny, nx)
+            f = dict(u=DeviceArray(shp, self.dtype), w1=DeviceArray(shp, self.dtype), w2=DeviceArray(shp, self.dtype),
+                     ctrl=DeviceArray((_capi.CTRL_BYTES,), np.uint8))
+            self.fields.append(f)
+            probs.append(_capi.Problem(f["u"].ptr, f["w1"].ptr, f["w2"].ptr, f["ctrl"].ptr, nx, ny, nz,
+                                       (hi - lo) / nx, (hi - lo) / ny, (hi - lo) / nz))
+        self.probs = (_capi.Problem * len(probs))(*probs)
+        self.batch = _capi.Batch()
+        self.workspace = None
+
+    @property
+    def npts(self) -> int:
+        return int(sum(nx * ny * nz for nx, ny, nz in self.sizes))
+
+    def set_initial(self, u0s, t0: float = 0.0, tend: float = float("inf"), kchunk: int = 0) -> None:
+        for f, u0 in zip(self.fields, u0s):
+            f["u"].upload(u0)
+        L = _capi.lib()
+        nbytes = L.mgad_batch_workspace_bytes(len(self.sizes), self.probs)
+        self.workspace = DeviceArray((nbytes,), np.uint8)
+        rc = L.mgad_batch_init(self.stream.handle, self.precision, ctypes.byref(self.scheme), len(self.sizes),
+                               self.probs, self.workspace.ptr, nbytes, kchunk, t0, tend, self.cfl,
+                               ctypes.byref(self.batch))
+        _capi.check(rc, "mgad_batch_init")
+
+    def steps(self, nsteps: int, graph: bool = True) -> None:
+        rc = _capi.lib().mgad_batch_steps(self.stream.handle, ctypes.byref(self.batch), nsteps, 1 if graph else 0)
+        _capi.check(rc, "mgad_batch_steps")
+
+    def steps_serial(self, nsteps: int, graph: bool = True) -> None:
+        rc = _capi.lib().mgad_batch_steps_serial(self.stream.handle, ctypes.byref(self.batch), self.probs, nsteps,
+                                                 1 if graph else 0)
+        _capi.check(rc, "mgad_batch_steps_serial")
+
+    def states(self):
+        out = []
+        for f in self.fields:
+            s = _capi.BurgersState()
+            rc = _capi.lib().mgad_burgers_read_ctrl(self.stream.handle, f["ctrl"].ptr, ctypes.byref(s))
+            _capi.check(rc, "mgad_burgers_read_ctrl")
+            out.append(BurgersState(s.t, s.dt, s.tend, s.cfl, s.amax, int(s.steps)))
+        return out
+
+    def solutions(self):
+        self.stream.synchronize()
+        return [f["u"].download() for f in self.fields]
+
+
+def heat7_params(shape, C: float = 1.0, L: float = 1.0, r: float = 0.15) -> dict:
+    """Node-inclusive spacing dx = L/(n-1) (SingleGPU/Diffusion3d_baselineCode/
+    main.cpp:55-57), coefficients kx = C/(12 dx^2) (:59-61), dt = r dx^2 / C."""
+    nx, ny, nz = (shape, shape, shape) if np.isscalar(shape) else shape
+    dx, dy, dz = L / (nx - 1), L / (ny - 1), L / (nz - 1)
+    return dict(nx=nx, ny=ny, nz=nz, dx=dx, dy=dy, dz=dz, kx=C / (12 * dx * dx), ky=C / (12 * dy * dy),
+                kz=C / (12 * dz * dz), dt=r * dx * dx / C)
+
+
 class Heat7:
     """u_t = C lap u, 7-point Laplacian (Compute_Laplace3d_O2) + forward Euler
     on the Dirichlet interior; kx = C/(12 dx^2) as SingleGPU/Diffusion3d_
     baselineCode/main.cpp:59-61 defines it."""
 
     def __init__(self, shape, C: float = 1.0, L: float = 1.0, r: float = 0.15, stream: Stream | None = None):
-        self.nx, self.ny, self.nz = (shape, shape, shape) if np.isscalar(shape) else shape
-        self.dx, self.dy, self.dz = L / (self.nx - 1), L / (self.ny - 1), L / (self.nz - 1)
-        self.kx = C / (12 * self.dx * self.dx)
-        self.ky = C / (12 * self.dy * self.dy)
-        self.kz = C / (12 * self.dz * self.dz)
-        self.dt = r * self.dx * self.dx / C
+        p = heat7_params(shape, C, L, r)
+        self.nx, self.ny, self.nz = p["nx"], p["ny"], p["nz"]
+        self.dx, self.dy, self.dz = p["dx"], p["dy"], p["dz"]
+        self.kx, self.ky, self.kz, self.dt = p["kx"], p["ky"], p["kz"], p["dt"]
         self.stream = stream if stream is not None else Stream()
         shp = (self.nz, self.ny, self.nx)
         self.u = DeviceArray(shp)

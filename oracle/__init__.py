@@ -89,7 +89,6 @@ def lib():
         L.orc_burgers_run.restype = ctypes.c_int
         L.orc_burgers_run.argtypes = [_U, _U, _U, _D, _D, _D, _D, _P, _D, ctypes.c_int, _S, _P]
         L.orc_heat7_fe.argtypes = [_U, _U, _U, _D, _D, _D, _D, ctypes.c_int, _P]
-        L.orc_matlab_heat3d.argtypes = [_U, _D, _D, _P, _P, _P, ctypes.POINTER(ctypes.c_int)]
         L.orc_rhs_1d_periodic.argtypes = [_U, _D, _P, _P, _S]
         L.orc_num_threads.restype = ctypes.c_int
         _lib = L
@@ -202,13 +201,6 @@ def burgers_run(u, d, cfl, t, tend, nsteps=-1, **kw):
 
 def heat7_fe(u, kx, ky, kz, dt, nsteps) -> None:
     lib().orc_heat7_fe(*_shape(u), kx, ky, kz, dt, nsteps, _p(u))
-
-
-def matlab_heat3d(nc, tfinal=0.5, factor=0.9):
-    l1, l2, li = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
-    it = ctypes.c_int()
-    lib().orc_matlab_heat3d(nc, tfinal, factor, ctypes.byref(l1), ctypes.byref(l2), ctypes.byref(li), ctypes.byref(it))
-    return l1.value, l2.value, li.value, it.value
 
 
 def rhs_1d_periodic(u, dx, **kw) -> np.ndarray:

@@ -29,10 +29,9 @@
  *                                       amplification 1 + z + z^2/2 + z^3/6.
  *  - orc_lu3d_o2, orc_heat7_fe ...... Dirichlet sine eigenmodes decay by exactly
  *                                       (1 - dt*lambda)^n; max principle.
- *  - orc_lu3d_13, orc_diff13_b5 ..... agree with the MATLAB-order 13-point operator,
- *                                       which orc_matlab_heat3d pins against the
- *                                       reference's recorded accuracy log
- *                                       (MLD/TestingAccuracy.log:66-72).
+ *  - orc_lu3d_13, orc_diff13_b5 ..... exact (to rounding) on polynomials of degree <= 5
+ *                                       in each direction (the 5-point 4th-order second
+ *                                       difference), and agree with each other.
  *  - orc_burgers_step/run ........... composition of the above; conservation of
  *                                       sum(u) over a periodic run.
  */
@@ -498,74 +497,6 @@ void orc_heat7_fe(unsigned nx, unsigned ny, unsigned nz, double kx, double ky,
     }
     free(Lu);
     free(uo);
-}
-
-/* MATLAB order-of-accuracy driver, MLD/diffusion3dTest.m:11-76 with
- * MLD/Laplace3d.m:32-60 ('OMP' strategy): 13-point operator with Dx/12
- * factors on nodes 3..n-2 (1-based), SSP-RK3, boundary layer reset after each
- * step, dt clipped after the step as the MATLAB loop does (:65-68). */
-void orc_matlab_heat3d(unsigned nc, double tfinal, double factor, double *l1,
-                       double *l2, double *linf, int *iters) {
-    const unsigned nx = nc, ny = nc, nz = nc;
-    const double D = 1.0, t0 = 0.1, L = 10;
-    const double dx = L / (nx - 1), dy = L / (ny - 1), dz = L / (nz - 1);
-    const double Dx = D / (dx * dx), Dy = D / (dy * dy), Dz = D / (dz * dz);
-    const size_t N = (size_t)nx * ny * nz, xy = (size_t)nx * ny;
-    double *u = (double *)malloc(sizeof(double) * N);
-    double *uo = (double *)malloc(sizeof(double) * N);
-    double *Lu = (double *)calloc(N, sizeof(double));
-    for (unsigned k = 0; k < nz; k++)
-        for (unsigned j = 0; j < ny; j++)
-            for (unsigned i = 0; i < nx; i++) {
-                double x = -L / 2 + i * dx, y = -L / 2 + j * dy, z = -L / 2 + k * dz;
-                u[i + nx * j + xy * k] = exp(-(x * x + y * y + z * z) / (4 * 1.0 * t0));
-            }
-    double dt = 1 / (2 * D * (1 / (dx * dx) + 1 / (dy * dy) + 1 / (dz * dz))) * factor;
-    double t = t0;
-    int it = 0;
-    while (t < tfinal) {
-        memcpy(uo, u, sizeof(double) * N);
-        for (unsigned stage = 1; stage <= 3; stage++) {
-            memset(Lu, 0, sizeof(double) * N);
-            for (unsigned k = 2; k < nz - 2; k++)
-                for (unsigned j = 2; j < ny - 2; j++)
-                    for (unsigned i = 2; i < nx - 2; i++) {
-                        size_t o = i + nx * j + xy * k;
-                        const size_t nx2 = 2 * (size_t)nx, xy2 = 2 * xy;
-                        double c = u[o];
-                        Lu[o] = Dx / 12 * (-u[o - 2] + 16 * u[o - 1] - 30 * c + 16 * u[o + 1] - u[o + 2]) +
-                                Dy / 12 * (-u[o - nx2] + 16 * u[o - nx] - 30 * c + 16 * u[o + nx] - u[o + nx2]) +
-                                Dz / 12 * (-u[o - xy2] + 16 * u[o - xy] - 30 * c + 16 * u[o + xy] - u[o + xy2]);
-                    }
-            for (size_t q = 0; q < N; q++) u[q] = rk_node(stage, dt, u[q], uo[q], Lu[q]);
-        }
-        for (unsigned k = 0; k < nz; k++)
-            for (unsigned j = 0; j < ny; j++)
-                for (unsigned i = 0; i < nx; i++)
-                    if (i == 0 || i == nx - 1 || j == 0 || j == ny - 1 || k == 0 || k == nz - 1)
-                        u[i + nx * j + xy * k] = 0;
-        if (t + dt > tfinal) dt = tfinal - t;
-        it++;
-        t = t + dt;
-    }
-    double s1 = 0, s2 = 0, mx = 0;
-    for (unsigned k = 0; k < nz; k++)
-        for (unsigned j = 0; j < ny; j++)
-            for (unsigned i = 0; i < nx; i++) {
-                double x = -L / 2 + i * dx, y = -L / 2 + j * dy, z = -L / 2 + k * dz;
-                double uE = pow(t0 / tfinal, 1.5) * exp(-(x * x + y * y + z * z) / (4 * 1.0 * tfinal));
-                double e = fabs(uE - u[i + nx * j + xy * k]);
-                s1 += e;
-                s2 += e * e;
-                if (e > mx) mx = e;
-            }
-    *l1 = dx * dy * dz * s1;
-    *l2 = sqrt(dx * dy * dz * s2);
-    *linf = mx;
-    *iters = it;
-    free(u);
-    free(uo);
-    free(Lu);
 }
 
 /* 1-D periodic derivative -(F_{i+1/2}-F_{i-1/2})/dx (order-of-accuracy checks). */

@@ -96,8 +96,6 @@ int    orc_burgers_run(unsigned nx, unsigned ny, unsigned nz, double dx,
 /* ---- heat ---- */
 void orc_heat7_fe(unsigned nx, unsigned ny, unsigned nz, double kx, double ky,
                   double kz, double dt, int nsteps, double *u);
-void orc_matlab_heat3d(unsigned nc, double tfinal, double factor, double *l1,
-                       double *l2, double *linf, int *iters);
 
 /* ---- 1-D periodic WENO derivative (order-of-accuracy checks) ---- */
 void orc_rhs_1d_periodic(unsigned n, double dx, const double *u, double *Lu,

@@ -1,0 +1,94 @@
+"""Batched persistent stages and launch configurations on the GPU.
+
+Every problem of a batch must end bit-identical to the same problem advanced
+alone (and to the CPU oracle), whatever tile width / z-chunk depth the launch
+uses, because each node's arithmetic does not depend on the decomposition.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from multigpu_advectiondiffusion_amd import solvers
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [16, 32, 48, (32, 16, 24), 16, 64, (48, 32, 8)]
+
+
+def alone(shape, u0, scheme, nsteps, tend=10.0):
+    s = solvers.PeriodicBurgers(shape, scheme=scheme)
+    s.set_initial(u0, tend=tend)
+    s.steps(nsteps, graph=False)
+    return s.solution(), s.state()
+
+
+@pytest.mark.parametrize("scheme", ["Z2", "JS"])
+@pytest.mark.parametrize("kchunk", [0, 8, 16])
+def test_batch_matches_single_problems(hipdev, scheme, kchunk):
+    b = solvers.BurgersBatch(SIZES, scheme=scheme)
+    u0s = [solvers.ic_fourier(s, seed=40 + i) for i, s in enumerate(SIZES)]
+    b.set_initial(u0s, tend=10.0, kchunk=kchunk)
+    assert b.batch.kchunk in (8, 16, 32) and b.batch.nitems > 0
+    b.steps(4, graph=True)
+    got = b.solutions()
+    sts = b.states()
+    for i, (shape, u0) in enumerate(zip(SIZES, u0s)):
+        ref, st = alone(shape, u0, scheme, 4)
+        assert np.array_equal(got[i], ref), (i, shape)
+        assert sts[i].steps == 4 and sts[i].t == st.t
+
+
+def test_batch_serial_path_and_oracle(hipdev):
+    b = solvers.BurgersBatch(SIZES[:4], scheme="Z2")
+    u0s = [solvers.ic_fourier(s, seed=60 + i) for i, s in enumerate(SIZES[:4])]
+    b.set_initial(u0s, tend=10.0)
+    b.steps_serial(3, graph=True)
+    got = b.solutions()
+    for i, (shape, u0) in enumerate(zip(SIZES[:4], u0s)):
+        ref = u0.copy()
+        nx, ny, nz = (shape,) * 3 if np.isscalar(shape) else shape
+        O.burgers_run(ref, (2.0 / nx, 2.0 / ny, 2.0 / nz), 0.4, 0.0, 10.0, nsteps=3, **O.Z2_GLOBAL)
+        assert np.array_equal(got[i], ref), i
+
+
+def test_batch_stops_each_problem_at_tend(hipdev):
+    sizes = [16, 32, 48]
+    b = solvers.BurgersBatch(sizes, scheme="JS")
+    u0s = [solvers.ic_fourier(s, seed=70 + i) for i, s in enumerate(sizes)]
+    b.set_initial(u0s, tend=0.2)
+    b.steps(200, graph=True)
+    sts = b.states()
+    got = b.solutions()
+    for i, s in enumerate(sizes):
+        assert sts[i].t == 0.2
+        ref = u0s[i].copy()
+        steps, t = O.burgers_run(ref, 2.0 / s, 0.4, 0.0, 0.2, **O.JS_GLOBAL)
+        assert sts[i].steps == steps
+        assert np.array_equal(got[i], ref)
+
+
+@pytest.mark.parametrize("tile,kchunk", [(16, 4), (32, 8), (16, 64), (32, 0), (0, 5)])
+def test_launch_configs_bit_identical(hipdev, tile, kchunk):
+    shape = (64, 32, 64)
+    u0 = solvers.ic_fourier(shape, seed=9)
+    ref, _ = alone(shape, u0, "Z2", 2)
+    solvers.set_launch("Z2", shape, tile, kchunk)
+    try:
+        got, _ = alone(shape, u0, "Z2", 2)
+    finally:
+        solvers.set_launch("Z2", shape, 0, 0)
+    assert np.array_equal(got, ref)
+
+
+def test_autotune(hipdev):
+    shape = 64
+    s = solvers.PeriodicBurgers(shape, scheme="Z2")
+    u0 = solvers.ic_fourier(shape, seed=11)
+    s.set_initial(u0, tend=10.0)
+    tile, kc = solvers.autotune(s)
+    assert tile in (16, 32) and 4 <= kc <= 64
+    s.set_initial(u0, tend=10.0)
+    s.steps(2)
+    ref, _ = alone(shape, u0, "Z2", 2)
+    assert np.array_equal(s.solution(), ref)
+    solvers.set_launch("Z2", shape, 0, 0)

@@ -130,3 +130,23 @@ def test_256_conservation(hipdev):
     m0, m1 = float(np.sum(u0)), float(np.sum(g))
     assert abs(m1 - m0) <= 1e-11 * np.sum(np.abs(u0))
     assert np.max(np.abs(g)) <= np.max(np.abs(u0)) * (1 + 1e-6)
+
+
+@pytest.mark.parametrize("scheme", ["Z2", "JS"])
+def test_fp32_storage_error_vs_fp64(hipdev, scheme):
+    """FP32 storage with FP64 weights: L1 / L-infinity distance to the FP64
+    run (bit-identical to the oracle) at t = 0.5 on 64^3, reported and bounded."""
+    n = 64
+    u0 = solvers.ic_fourier(n, seed=3)
+    a = solvers.PeriodicBurgers(n, scheme=scheme, precision="f64")
+    b = solvers.PeriodicBurgers(n, scheme=scheme, precision="f32")
+    a.set_initial(u0, tend=0.5)
+    b.set_initial(u0, tend=0.5)
+    sa, sb = a.run(batch=16), b.run(batch=16)
+    assert sa.t == 0.5 and sb.t == 0.5
+    ua, ub = a.solution(), b.solution().astype(np.float64)
+    l1 = float(np.mean(np.abs(ub - ua)))
+    linf = float(np.max(np.abs(ub - ua)))
+    print(f"fp32 vs fp64 {scheme} 64^3 t=0.5: L1 {l1:.3e}  Linf {linf:.3e}  steps {sa.steps}/{sb.steps}")
+    assert np.isfinite(ub).all()
+    assert l1 < 1e-5 and linf < 1e-2
