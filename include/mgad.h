@@ -257,6 +257,36 @@ int mgad_batch_steps_serial(hipStream_t stream, const mgad_batch *batch,
                             const mgad_problem *probs, int nsteps,
                             int use_graph);
 
+/* ------------------------------------------------------------------------ */
+/* Discrete adjoint of the periodic Burgers run (FP64)                       */
+/* ------------------------------------------------------------------------ */
+
+/* Device bytes mgad_burgers_adjoint needs for a grid, a bound on the number
+ * of steps and a checkpoint interval (sqrt(max_steps) keeps both the
+ * checkpoints and one recomputed segment at about sqrt(N) fields). */
+size_t mgad_adjoint_workspace_bytes(unsigned int nx, unsigned int ny,
+                                    unsigned int nz, int max_steps,
+                                    int checkpoint_every);
+
+/* J = 1/2 |u(tend) - target|^2 for the run of mgad_burgers_steps from u0 at
+ * t0 (CFL rule, global LF alpha per stage) and its gradient dJ/du0 by the
+ * discrete adjoint: the forward pass stores u every checkpoint_every steps;
+ * the reverse sweep recomputes each segment from its checkpoint (keeping the
+ * stage inputs) and applies the stage VJPs in reverse.  The alpha = max|u|
+ * dependence is differentiated at the first node attaining the maximum; dt
+ * = cfl*dx/max|u^n| and the clipped last step dt = tend - t are included.
+ * u0, target, grad: device, nx*ny*nz doubles; nx, ny multiples of 16.
+ * Synchronises the stream; returns MGAD_ERR_INVALID_ARG if the run needs
+ * more than max_steps steps. */
+int mgad_burgers_adjoint(hipStream_t stream, const mgad_scheme *scheme,
+                         unsigned int nx, unsigned int ny, unsigned int nz,
+                         double dx, double dy, double dz, double cfl,
+                         double t0, double tend, const double *u0,
+                         const double *target, double *grad, double *J,
+                         int max_steps, int checkpoint_every,
+                         void *workspace, size_t workspace_bytes,
+                         int *nsteps);
+
 /* Copies ctrl to the host (synchronises the stream). */
 int mgad_burgers_read_ctrl(hipStream_t stream, const void *ctrl,
                            mgad_burgers_state *out);

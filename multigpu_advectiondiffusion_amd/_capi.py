@@ -26,6 +26,7 @@ EXPORTS = (
     "mgad_burgers_init", "mgad_burgers_stage", "mgad_burgers_steps", "mgad_burgers_read_ctrl",
     "mgad_burgers_init_p", "mgad_burgers_steps_p", "mgad_burgers_autotune", "mgad_burgers_set_launch",
     "mgad_batch_workspace_bytes", "mgad_batch_init", "mgad_batch_steps", "mgad_batch_steps_serial",
+    "mgad_adjoint_workspace_bytes", "mgad_burgers_adjoint",
     "mgad_heat7_fe_steps", "mgad_ic_fourier", "mgad_ic_gauss_noise",
 )
 
@@ -121,6 +122,11 @@ def lib():
         L.mgad_batch_steps.restype = I
         L.mgad_batch_steps_serial.argtypes = [VP, BP, PP, I, I]
         L.mgad_batch_steps_serial.restype = I
+        L.mgad_adjoint_workspace_bytes.argtypes = [U, U, U, I, I]
+        L.mgad_adjoint_workspace_bytes.restype = ctypes.c_size_t
+        L.mgad_burgers_adjoint.argtypes = [VP, ctypes.POINTER(Scheme), U, U, U, D, D, D, D, D, D, VP, VP, VP,
+                                           ctypes.POINTER(D), I, I, VP, ctypes.c_size_t, IP]
+        L.mgad_burgers_adjoint.restype = I
         L.mgad_burgers_read_ctrl.argtypes = [VP, VP, ctypes.POINTER(BurgersState)]
         L.mgad_burgers_read_ctrl.restype = I
         L.mgad_heat7_fe_steps.argtypes = [VP, U, U, U, D, D, D, D, VP, VP, I, I]

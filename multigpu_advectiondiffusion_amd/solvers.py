@@ -201,6 +201,40 @@ class BurgersBatch:
         return [f["u"].download() for f in self.fields]
 
 
+class BurgersAdjoint:
+    """dJ/du0 of J = 1/2 |u(tend) - target|^2 for the periodic Burgers run
+    (WENO5 + global LF + SSP-RK3, CFL dt) by the library's discrete adjoint
+    with checkpoints every `every` steps (default ceil(sqrt(max_steps)))."""
+
+    def __init__(self, shape, scheme: str = "Z2", cfl: float = 0.4, max_steps: int = 1000, every: int | None = None,
+                 lo: float = -1.0, hi: float = 1.0, stream: Stream | None = None):
+        self.nx, self.ny, self.nz = (shape, shape, shape) if np.isscalar(shape) else shape
+        self.dx, self.dy, self.dz = (hi - lo) / self.nx, (hi - lo) / self.ny, (hi - lo) / self.nz
+        self.scheme = scheme_struct(scheme)
+        self.cfl = cfl
+        self.max_steps = max_steps
+        self.every = every if every else int(np.ceil(np.sqrt(max_steps)))
+        self.stream = stream if stream is not None else Stream()
+        shp = (self.nz, self.ny, self.nx)
+        self.u0, self.target, self.grad = DeviceArray(shp), DeviceArray(shp), DeviceArray(shp)
+        nbytes = _capi.lib().mgad_adjoint_workspace_bytes(self.nx, self.ny, self.nz, max_steps, self.every)
+        if nbytes == 0:
+            raise _capi.MgadError("mgad_adjoint_workspace_bytes: invalid arguments")
+        self.workspace = DeviceArray((nbytes,), np.uint8)
+
+    def __call__(self, u0: np.ndarray, target: np.ndarray, tend: float, t0: float = 0.0):
+        """Returns (J, dJ/du0, steps)."""
+        self.u0.upload(u0)
+        self.target.upload(target)
+        J, n = ctypes.c_double(), ctypes.c_int()
+        rc = _capi.lib().mgad_burgers_adjoint(self.stream.handle, ctypes.byref(self.scheme), self.nx, self.ny,
+                                              self.nz, self.dx, self.dy, self.dz, self.cfl, t0, tend, self.u0.ptr,
+                                              self.target.ptr, self.grad.ptr, ctypes.byref(J), self.max_steps,
+                                              self.every, self.workspace.ptr, self.workspace.nbytes, ctypes.byref(n))
+        _capi.check(rc, "mgad_burgers_adjoint")
+        return J.value, self.grad.download(), n.value
+
+
 def heat7_params(shape, C: float = 1.0, L: float = 1.0, r: float = 0.15) -> dict:
     """Node-inclusive spacing dx = L/(n-1) (SingleGPU/Diffusion3d_baselineCode/
     main.cpp:55-57), coefficients kx = C/(12 dx^2) (:59-61), dt = r dx^2 / C."""

@@ -2,6 +2,7 @@
 
     python -m multigpu_advectiondiffusion_amd.studies batched   [--steps 200]
     python -m multigpu_advectiondiffusion_amd.studies fp32      [--n 512]
+    python -m multigpu_advectiondiffusion_amd.studies adjoint   [--n 128]
 
 batched: 64 independent periodic WENO5-Z (p=2, eps=1e-40) FP64 problems with
     edge lengths drawn seeded-uniformly from {32,48,64,96,128} (SplitMix64 seed
@@ -14,6 +15,14 @@ fp32: the same Burgers setup with WENO5-Z at n^3, run to t = 0.5 once in FP64
     and once with FP32 storage (FP64 split fluxes, smoothness indicators and
     weights); reports the L1 (mean |diff|) and L-infinity error of the FP32
     result against the FP64 one.
+adjoint: J = 1/2 |u(T) - u_target|^2 for WENO5-Z (p=2) Burgers at n^3 FP64,
+    u0 the seed-5 Fourier field and u_target the forward solution from u0 +
+    1e-2 * uniform(-1,1) noise (numpy seed 5).  Cost: wall time of the
+    gradient (forward with sqrt(N) checkpoints + recompute + reverse) over
+    the plain forward run, on the run of 400 steps.  Accuracy: the gradient
+    along 16 seeded unit directions against central finite differences
+    (h = 1e-6, formed point-wise) at T = 0.2 (before shocks) and T = 0.5
+    (after).
 Each prints one JSON line.
 """
 from __future__ import annotations
@@ -122,14 +131,77 @@ def fp32(n: int = 512, tend: float = 0.5, seed: int = 3) -> dict:
     return out
 
 
+def adjoint(n: int = 128, steps: int = 400, ndir: int = 16, h: float = 1e-6) -> dict:
+    u0 = solvers.ic_fourier(n, seed=5)
+    noise = np.random.default_rng(5).uniform(-1.0, 1.0, size=u0.shape)
+    out = {"study": "adjoint", "grid": n, "scheme": "WENO5-Z p=2 eps=1e-40", "fd_h": h, "directions": ndir}
+
+    def forward(u, tend, stream=None):
+        s = solvers.PeriodicBurgers(n, scheme="Z2", stream=stream)
+        s.set_initial(u, tend=tend)
+        st = s.run(batch=64)
+        return s.solution(), st
+
+    # cost on the `steps`-step run: find its end time, then time both paths
+    s = solvers.PeriodicBurgers(n, scheme="Z2")
+    s.set_initial(u0, tend=float("inf"))
+    s.steps(steps)
+    T = s.state().t
+    target, _ = forward(u0 + 1e-2 * noise, T)
+    adj = solvers.BurgersAdjoint(n, scheme="Z2", max_steps=steps + 8, stream=s.stream)
+    adj(u0, target, T)  # warm-up (graphs, caches)
+    e0, e1 = hip.Event(), hip.Event()
+    hip.synchronize()
+    e0.record(s.stream)
+    s.set_initial(u0, tend=T)
+    st = s.run(batch=steps + 8)
+    e1.record(s.stream)
+    e1.synchronize()
+    t_fwd = e0.elapsed_ms(e1) / 1e3
+    hip.synchronize()
+    e0.record(s.stream)
+    J, g, nst = adj(u0, target, T)
+    e1.record(s.stream)
+    e1.synchronize()
+    t_adj = e0.elapsed_ms(e1) / 1e3
+    out.update({"cost_steps": nst, "cost_T": T, "forward_s": t_fwd, "adjoint_s": t_adj,
+                "cost_ratio": t_adj / t_fwd, "checkpoint_every": adj.every})
+    rng = np.random.default_rng(2026)
+    for tend in (0.2, 0.5):
+        tg, _ = forward(u0 + 1e-2 * noise, tend)
+        J, g, nst = adj(u0, tg, tend)
+        gnorm = float(np.linalg.norm(g))
+        res = {"steps": nst, "J": J, "grad_norm": gnorm}
+        dirs = []
+        for _ in range(ndir):
+            v = rng.standard_normal(u0.shape)
+            v /= np.linalg.norm(v)
+            dirs.append(v)
+        for hh in (h, 10 * h):
+            rel, reln = [], []
+            for v in dirs:
+                up, _ = forward(u0 + hh * v, tend)
+                um, _ = forward(u0 - hh * v, tend)
+                fd = float(np.sum(0.5 * (up - um) * (up + um - 2 * tg)) / (2 * hh))
+                ad = float(np.sum(g * v))
+                rel.append(abs(fd - ad) / abs(ad))
+                reln.append(abs(fd - ad) / gnorm)
+            res[f"h{hh:g}"] = {"max_rel_err": max(rel), "median_rel_err": float(np.median(rel)),
+                               "max_err_over_grad_norm": max(reln)}
+        out[f"T{tend}"] = res
+    return out
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
-    ap.add_argument("study", choices=["batched", "fp32"])
+    ap.add_argument("study", choices=["batched", "fp32", "adjoint"])
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--n", type=int, default=512)
     args = ap.parse_args(argv)
     if args.study == "batched":
         r = batched(args.steps)
+    elif args.study == "adjoint":
+        r = adjoint(args.n if args.n != 512 else 128)
     else:
         r = fp32(args.n)
     print(json.dumps(r), flush=True)

@@ -90,6 +90,12 @@ def lib():
         L.orc_burgers_run.argtypes = [_U, _U, _U, _D, _D, _D, _D, _P, _D, ctypes.c_int, _S, _P]
         L.orc_heat7_fe.argtypes = [_U, _U, _U, _D, _D, _D, _D, ctypes.c_int, _P]
         L.orc_rhs_1d_periodic.argtypes = [_U, _D, _P, _P, _S]
+        L.orc_side_grad.restype = _D
+        L.orc_side_grad.argtypes = [_P, ctypes.c_int, _S, _P]
+        L.orc_rhs_vjp.argtypes = [_U, _U, _U, _D, _D, _D, _P, _P, _P, _S, _P, _P]
+        L.orc_burgers_adjoint.restype = _D
+        L.orc_burgers_adjoint.argtypes = [_U, _U, _U, _D, _D, _D, _D, _D, _D, _S, _P, _P, _P,
+                                          ctypes.POINTER(ctypes.c_int)]
         L.orc_num_threads.restype = ctypes.c_int
         _lib = L
     return _lib
@@ -209,3 +215,35 @@ def rhs_1d_periodic(u, dx, **kw) -> np.ndarray:
     s = scheme(**kw)
     lib().orc_rhs_1d_periodic(u.size, dx, _p(u), _p(Lu), ctypes.byref(s))
     return Lu
+
+
+def side_grad(v, plus=True, **kw):
+    """(h, dh/dv) of one reconstruction side."""
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    g = np.empty(5)
+    sc = scheme(**kw)
+    h = lib().orc_side_grad(_p(v), 1 if plus else 0, ctypes.byref(sc), _p(g))
+    return h, g
+
+
+def rhs_vjp(u, lam, d, alpha, **kw):
+    """((dL/du)^T lam, <lam, L>, <lam, dL/dalpha>) for the periodic global-LF RHS."""
+    dx, dy, dz = (d, d, d) if np.isscalar(d) else d
+    ub = np.zeros_like(u)
+    L, La = ctypes.c_double(), ctypes.c_double()
+    sc = scheme(**kw, alpha=alpha)
+    sc.lf = LF_GLOBAL
+    lib().orc_rhs_vjp(*_shape(u), dx, dy, dz, _p(u), _p(np.ascontiguousarray(lam)), _p(ub), ctypes.byref(sc),
+                      ctypes.byref(L), ctypes.byref(La))
+    return ub, L.value, La.value
+
+
+def burgers_adjoint(u0, target, d, cfl, t0, tend, **kw):
+    """(J, dJ/du0, steps) of J = 1/2 |u(tend) - target|^2."""
+    dx, dy, dz = (d, d, d) if np.isscalar(d) else d
+    g = np.empty_like(u0)
+    n = ctypes.c_int()
+    sc = scheme(**kw)
+    J = lib().orc_burgers_adjoint(*_shape(u0), dx, dy, dz, cfl, t0, tend, ctypes.byref(sc), _p(u0),
+                                  _p(np.ascontiguousarray(target)), _p(g), ctypes.byref(n))
+    return J, g, n.value

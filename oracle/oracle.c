@@ -518,3 +518,240 @@ void orc_rhs_1d_periodic(unsigned n, double dx, const double *u, double *Lu,
     free(fm);
     free(F);
 }
+
+/* ------------------------------------------------------------------------ */
+/* Discrete adjoint (reverse mode) of the periodic global-LF Burgers step.  */
+/* ------------------------------------------------------------------------ */
+/* Derivative of the face-flux side function of orc_weno_plus / orc_weno_minus
+ * (B5/kernels.cu:142-192 for WENO-Z, MB3/Kernels.cu:112-220 for WENO-JS),
+ * reverse mode with unit output seed.  The JS normalisation w = alpha/sum,
+ * (sum w c)/6, is the same function as (sum alpha c)/(6 sum). */
+double orc_side_grad(const double *v, int plus, const orc_scheme *s, double *g) {
+    const double eps = s->eps;
+    const double s0 = (v[0] - 2 * v[1]) + v[2], t0 = (v[0] - 4 * v[1]) + 3 * v[2];
+    const double s1 = (v[1] - 2 * v[2]) + v[3], t1 = v[1] - v[3];
+    const double s2 = (v[2] - 2 * v[3]) + v[4], t2 = (3 * v[2] - 4 * v[3]) + v[4];
+    const double B[3] = {C1312 * s0 * s0 + C14 * t0 * t0, C1312 * s1 * s1 + C14 * t1 * t1,
+                         C1312 * s2 * s2 + C14 * t2 * t2};
+    const double d[3] = {plus ? D0N : D0P, plus ? D1N : D1P, plus ? D2N : D2P};
+    double c[3];
+    if (plus) {
+        c[0] = 2 * v[0] - 7 * v[1] + 11 * v[2];
+        c[1] = -v[1] + 5 * v[2] + 2 * v[3];
+        c[2] = 2 * v[2] + 5 * v[3] - v[4];
+    } else {
+        c[0] = -v[0] + 5 * v[1] + 2 * v[2];
+        c[1] = 2 * v[1] + 5 * v[2] - v[3];
+        c[2] = 11 * v[2] - 7 * v[3] + 2 * v[4];
+    }
+    const int Z = (s->weno == ORC_WENO_Z || s->weno == ORC_WENO_Z_B5S);
+    double a[3], b[3], q[3], tau = 0;
+    if (Z) tau = fabs(B[0] - B[2]);
+    for (int k = 0; k < 3; k++) {
+        b[k] = B[k] + eps;
+        if (Z) {
+            q[k] = tau / b[k];
+            a[k] = d[k] * (1. + zpow(q[k], s->p));
+        } else {
+            a[k] = d[k] / (b[k] * b[k]);
+        }
+    }
+    const double S = a[0] + a[1] + a[2], N = a[0] * c[0] + a[1] * c[1] + a[2] * c[2];
+    const double h = N / (6 * S);
+    /* reverse */
+    const double Nb = 1.0 / (6 * S), Sb = -h / S;
+    double ab[3], cb[3], Bb[3] = {0, 0, 0}, taub = 0;
+    for (int k = 0; k < 3; k++) {
+        ab[k] = Nb * c[k] + Sb;
+        cb[k] = Nb * a[k];
+        if (Z) {
+            const double qb = s->p == 2 ? ab[k] * d[k] * 2 * q[k] : ab[k] * d[k];
+            taub += qb / b[k];
+            Bb[k] += -qb * q[k] / b[k];
+        } else {
+            Bb[k] += ab[k] * (-2 * a[k] / b[k]);
+        }
+    }
+    if (Z) {
+        const double sg = (B[0] - B[2]) >= 0 ? 1.0 : -1.0;
+        Bb[0] += sg * taub;
+        Bb[2] -= sg * taub;
+    }
+    const double s0b = 2 * C1312 * s0 * Bb[0], t0b = 2 * C14 * t0 * Bb[0];
+    const double s1b = 2 * C1312 * s1 * Bb[1], t1b = 2 * C14 * t1 * Bb[1];
+    const double s2b = 2 * C1312 * s2 * Bb[2], t2b = 2 * C14 * t2 * Bb[2];
+    g[0] = s0b + t0b;
+    g[1] = -2 * s0b - 4 * t0b + s1b + t1b;
+    g[2] = s0b + 3 * t0b - 2 * s1b + s2b + 3 * t2b;
+    g[3] = s1b - t1b - 2 * s2b - 4 * t2b;
+    g[4] = s2b + t2b;
+    if (plus) {
+        g[0] += 2 * cb[0];
+        g[1] += -7 * cb[0] - cb[1];
+        g[2] += 11 * cb[0] + 5 * cb[1] + 2 * cb[2];
+        g[3] += 2 * cb[1] + 5 * cb[2];
+        g[4] += -cb[2];
+    } else {
+        g[0] += -cb[0];
+        g[1] += 5 * cb[0] + 2 * cb[1];
+        g[2] += 2 * cb[0] + 5 * cb[1] + 11 * cb[2];
+        g[3] += -cb[1] - 7 * cb[2];
+        g[4] += 2 * cb[2];
+    }
+    return h;
+}
+
+/* ubar += (dL/du)^T lam along one axis (line by line, serial per line so the
+ * accumulation order is fixed). */
+static void vjp_axis(unsigned nx, unsigned ny, unsigned nz, int ax, double d, const double *u,
+                     const double *lam, double *ubar, const orc_scheme *s, double *lamL, double *lamLa) {
+    const size_t xy = (size_t)nx * ny;
+    const unsigned n = ax == 0 ? nx : (ax == 1 ? ny : nz);
+    const unsigned na = ax == 0 ? ny : nx, nb = ax == 2 ? ny : nz;
+    double sL = 0, sLa = 0;
+#pragma omp parallel for collapse(2) reduction(+ : sL, sLa) schedule(static)
+    for (unsigned b = 0; b < nb; b++)
+        for (unsigned a = 0; a < na; a++) {
+            size_t base, st;
+            if (ax == 0) { base = (size_t)nx * a + xy * b; st = 1; }
+            else if (ax == 1) { base = a + xy * b; st = nx; }
+            else { base = a + (size_t)nx * b; st = xy; }
+            double *fp = (double *)malloc(sizeof(double) * n * 4);
+            double *fm = fp + n, *fpb = fp + 2 * n, *fmb = fp + 3 * n;
+            for (unsigned i = 0; i < n; i++) {
+                orc_split(u[base + st * i], s->alpha, ORC_LF_GLOBAL, &fp[i], &fm[i]);
+                fpb[i] = fmb[i] = 0;
+            }
+            for (unsigned i = 0; i < n; i++) {
+                double v[5], w[5], gv[5], gw[5];
+                for (int q = 0; q < 5; q++) {
+                    v[q] = fp[wrap((long)i - 2 + q, n)];
+                    w[q] = fm[wrap((long)i - 1 + q, n)];
+                }
+                const double F = orc_side_grad(v, 1, s, gv) + orc_side_grad(w, 0, s, gw);
+                const double Fb = (lam[base + st * wrap((long)i + 1, n)] - lam[base + st * i]) / d;
+                sL += F * Fb;
+                for (int q = 0; q < 5; q++) {
+                    fpb[wrap((long)i - 2 + q, n)] += Fb * gv[q];
+                    fmb[wrap((long)i - 1 + q, n)] += Fb * gw[q];
+                }
+            }
+            for (unsigned j = 0; j < n; j++) {
+                const double uj = u[base + st * j];
+                ubar[base + st * j] += fpb[j] * (0.5 * (uj + s->alpha)) + fmb[j] * (0.5 * (uj - s->alpha));
+                sLa += 0.5 * uj * (fpb[j] - fmb[j]);
+            }
+            free(fp);
+        }
+    *lamL += sL;
+    *lamLa += sLa;
+}
+
+void orc_rhs_vjp(unsigned nx, unsigned ny, unsigned nz, double dx, double dy, double dz,
+                 const double *u, const double *lam, double *ubar, const orc_scheme *s,
+                 double *lam_dot_L, double *lam_dot_La) {
+    double L = 0, La = 0;
+    vjp_axis(nx, ny, nz, 0, dx, u, lam, ubar, s, &L, &La);
+    vjp_axis(nx, ny, nz, 1, dy, u, lam, ubar, s, &L, &La);
+    vjp_axis(nx, ny, nz, 2, dz, u, lam, ubar, s, &L, &La);
+    *lam_dot_L = L;
+    *lam_dot_La = La;
+}
+
+/* index of the first node with the largest |u| (the subgradient used for
+ * alpha = max|u|) */
+static size_t argmax_abs(size_t n, const double *u) {
+    size_t im = 0;
+    double m = -1;
+    for (size_t q = 0; q < n; q++)
+        if (fabs(u[q]) > m) { m = fabs(u[q]); im = q; }
+    return im;
+}
+
+double orc_burgers_adjoint(unsigned nx, unsigned ny, unsigned nz, double dx, double dy, double dz,
+                           double cfl, double t0, double tend, const orc_scheme *s0,
+                           const double *u0, const double *target, double *grad, int *nsteps) {
+    const size_t N = (size_t)nx * ny * nz;
+    orc_scheme s = *s0;
+    s.lf = ORC_LF_GLOBAL;
+    /* forward, keeping every stage input */
+    int cap = 64, n = 0;
+    double **st = (double **)malloc(sizeof(double *) * cap * 3);
+    double *dts = (double *)malloc(sizeof(double) * cap), *al = (double *)malloc(sizeof(double) * cap * 3);
+    int *clip = (int *)malloc(sizeof(int) * cap);
+    double *u = (double *)malloc(sizeof(double) * N), *Lu = (double *)malloc(sizeof(double) * N);
+    memcpy(u, u0, sizeof(double) * N);
+    double t = t0;
+    while (t < tend) {
+        if (n == cap) {
+            cap *= 2;
+            st = (double **)realloc(st, sizeof(double *) * cap * 3);
+            dts = (double *)realloc(dts, sizeof(double) * cap);
+            al = (double *)realloc(al, sizeof(double) * cap * 3);
+            clip = (int *)realloc(clip, sizeof(int) * cap);
+        }
+        const double a1 = orc_maxabs(N, u);
+        double dt = cfl * dx / a1;
+        clip[n] = 0;
+        if (t + dt > tend) { dt = tend - t; clip[n] = 1; }
+        dts[n] = dt;
+        double *un = (double *)malloc(sizeof(double) * N);
+        memcpy(un, u, sizeof(double) * N);
+        for (unsigned stage = 1; stage <= 3; stage++) {
+            double *in = (double *)malloc(sizeof(double) * N);
+            memcpy(in, u, sizeof(double) * N);
+            st[3 * n + stage - 1] = in;
+            s.alpha = stage == 1 ? a1 : orc_maxabs(N, u);
+            al[3 * n + stage - 1] = s.alpha;
+            orc_rhs_periodic(nx, ny, nz, dx, dy, dz, u, Lu, &s);
+            orc_rk_stage_all(N, stage, dt, u, un, Lu);
+        }
+        free(un);
+        t = t + dt;
+        n++;
+    }
+    double J = 0;
+    double *ub = (double *)malloc(sizeof(double) * N), *ub2 = (double *)malloc(sizeof(double) * N);
+    double *ub1 = (double *)malloc(sizeof(double) * N), *acc = (double *)malloc(sizeof(double) * N);
+    double *lam = (double *)malloc(sizeof(double) * N);
+    for (size_t q = 0; q < N; q++) {
+        const double e = u[q] - target[q];
+        J += 0.5 * e * e;
+        ub[q] = e;
+    }
+    double carry = 0;
+    for (int m = n - 1; m >= 0; m--) {
+        const double dt = dts[m];
+        double dtb = 0, L, La;
+        /* stage 3: u3 = (un + 2 (u2 + dt L(u2)))/3 */
+        for (size_t q = 0; q < N; q++) { lam[q] = (2.0 / 3.0) * dt * ub[q]; ub2[q] = (2.0 / 3.0) * ub[q]; acc[q] = ub[q] / 3.0; }
+        s.alpha = al[3 * m + 2];
+        orc_rhs_vjp(nx, ny, nz, dx, dy, dz, st[3 * m + 2], lam, ub2, &s, &L, &La);
+        dtb += L / dt;
+        { size_t im = argmax_abs(N, st[3 * m + 2]); ub2[im] += La * (st[3 * m + 2][im] >= 0 ? 1.0 : -1.0); }
+        /* stage 2: u2 = 0.75 un + 0.25 (u1 + dt L(u1)) */
+        for (size_t q = 0; q < N; q++) { lam[q] = 0.25 * dt * ub2[q]; ub1[q] = 0.25 * ub2[q]; acc[q] += 0.75 * ub2[q]; }
+        s.alpha = al[3 * m + 1];
+        orc_rhs_vjp(nx, ny, nz, dx, dy, dz, st[3 * m + 1], lam, ub1, &s, &L, &La);
+        dtb += L / dt;
+        { size_t im = argmax_abs(N, st[3 * m + 1]); ub1[im] += La * (st[3 * m + 1][im] >= 0 ? 1.0 : -1.0); }
+        /* stage 1: u1 = un + dt L(un) */
+        for (size_t q = 0; q < N; q++) { lam[q] = dt * ub1[q]; ub[q] = acc[q] + ub1[q]; }
+        s.alpha = al[3 * m];
+        orc_rhs_vjp(nx, ny, nz, dx, dy, dz, st[3 * m], lam, ub, &s, &L, &La);
+        dtb += L / dt;
+        double a1b = La;
+        if (clip[m]) {
+            carry = dtb;  /* dt_last = tend - sum of earlier dt */
+        } else {
+            a1b += -(dtb - carry) * dt / al[3 * m];
+        }
+        { size_t im = argmax_abs(N, st[3 * m]); ub[im] += a1b * (st[3 * m][im] >= 0 ? 1.0 : -1.0); }
+    }
+    if (grad) memcpy(grad, ub, sizeof(double) * N);
+    for (int q = 0; q < 3 * n; q++) free(st[q]);
+    free(st); free(dts); free(al); free(clip); free(u); free(Lu);
+    free(ub); free(ub2); free(ub1); free(acc); free(lam);
+    if (nsteps) *nsteps = n;
+    return J;
+}

@@ -101,6 +101,23 @@ void orc_heat7_fe(unsigned nx, unsigned ny, unsigned nz, double kx, double ky,
 void orc_rhs_1d_periodic(unsigned n, double dx, const double *u, double *Lu,
                          const orc_scheme *s);
 
+/* ---- discrete adjoint of the periodic global-LF Burgers step ---- */
+/* Value and gradient of one side of the reconstruction w.r.t. its 5 inputs
+ * (plus: f+ side, weights (1,6,3)/10; else the f- side). */
+double orc_side_grad(const double *v, int plus, const orc_scheme *s, double *g);
+/* VJP of L(u; alpha) (orc_rhs_periodic) applied to lam, ADDED into ubar;
+ * returns <lam, L(u)> in *lam_dot_L and <lam, dL/dalpha> in *lam_dot_La. */
+void orc_rhs_vjp(unsigned nx, unsigned ny, unsigned nz, double dx, double dy,
+                 double dz, const double *u, const double *lam, double *ubar,
+                 const orc_scheme *s, double *lam_dot_L, double *lam_dot_La);
+/* J = 1/2 |u(T) - target|^2 after a run from t0 to tend (CFL rule,
+ * global alpha per stage); returns J, writes dJ/du0 into grad (may be NULL)
+ * and the number of steps into *nsteps. */
+double orc_burgers_adjoint(unsigned nx, unsigned ny, unsigned nz, double dx,
+                           double dy, double dz, double cfl, double t0,
+                           double tend, const orc_scheme *s, const double *u0,
+                           const double *target, double *grad, int *nsteps);
+
 int  orc_num_threads(void);
 
 #ifdef __cplusplus
