@@ -6,7 +6,14 @@ grid; inputs are resident in HBM before the timed region starts.  Prints one
 JSON line (the contract the harness reads).
 
     python bench.py [--steps K] [--warmup W] [--n 256] [--scheme Z2|JS]
-                    [--no-cpu] [--gpus 1]
+                    [--no-cpu] [--quick] [--gpus 1]
+
+After the timed region (and the CPU baseline) it also runs the three other
+measurements the metric names, reported as extra fields: the batched
+uneven-size speedup over one-by-one launches (64 problems, 200 steps), the
+adjoint cost ratio and finite-difference agreement (128^3, 400 steps / 16
+directions), and the FP32-storage error against FP64 (512^3 to t = 0.5);
+--quick skips them.  Progress goes to stderr.
 """
 from __future__ import annotations
 
@@ -76,6 +83,7 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--quick", action="store_true", help="skip the batched / adjoint / fp32 studies")
     ap.add_argument("--copy-calib", action="store_true",
                     help="after timing, run 3 device copies of one field (counter calibration)")
     args = ap.parse_args()
@@ -153,8 +161,27 @@ def main() -> int:
         for _ in range(3):
             s.w1.copy_from(s.u, s.stream)
         s.stream.synchronize()
+    print(f"timed {args.steps} steps: {ms / args.steps:.3f} ms/step, {gpts:.3f} Gpts/s", file=sys.stderr, flush=True)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(n, args.scheme, args.seed)
+        print("cpu baseline done", file=sys.stderr, flush=True)
+    if not args.quick:
+        from multigpu_advectiondiffusion_amd import studies
+
+        b = studies.batched(steps=200)
+        out["batched"] = {k: b[k] for k in ("speedup", "batched_gpts", "one_by_one_gpts", "problems", "points",
+                                             "steps", "results_identical")}
+        print("batched study done", file=sys.stderr, flush=True)
+        a = studies.adjoint(n=128)
+        out["adjoint"] = {"cost_ratio": a["cost_ratio"], "steps": a["cost_steps"], "grid": 128,
+                          "fd_max_err_over_grad_norm_h1e-6": max(a["T0.2"]["h1e-06"]["max_err_over_grad_norm"],
+                                                                 a["T0.5"]["h1e-06"]["max_err_over_grad_norm"]),
+                          "fd_median_rel_err_h1e-6": max(a["T0.2"]["h1e-06"]["median_rel_err"],
+                                                         a["T0.5"]["h1e-06"]["median_rel_err"])}
+        print("adjoint study done", file=sys.stderr, flush=True)
+        f = studies.fp32(n=512)
+        out["fp32_storage_512"] = {k: f[k] for k in ("l1", "linf", "steps_f64", "gpts_f64", "gpts_f32")}
+        print("fp32 study done", file=sys.stderr, flush=True)
     print(json.dumps(out), flush=True)
     return 0
 

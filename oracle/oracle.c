@@ -34,6 +34,15 @@
  *                                       difference), and agree with each other.
  *  - orc_burgers_step/run ........... composition of the above; conservation of
  *                                       sum(u) over a periodic run.
+ *  - orc_side_grad .................. central finite differences of orc_face per
+ *                                       stencil entry (test_side_gradient_fd).
+ *  - orc_rhs_vjp .................... dot-product test <lam, J v> = <J^T lam, v> with
+ *                                       J v from central differences of orc_rhs_periodic
+ *                                       at fixed alpha, and the alpha derivative against
+ *                                       central differences in alpha (test_rhs_vjp_dot_product).
+ *  - orc_burgers_adjoint ............ central finite differences of J along seeded
+ *                                       directions, before and after shocks
+ *                                       (test_adjoint_fd).
  */
 #include "oracle.h"
 
