@@ -305,6 +305,20 @@ int mgad_heat7_fe_steps(hipStream_t stream, unsigned int nx, unsigned int ny,
                         int use_graph);
 
 /* ------------------------------------------------------------------------ */
+/* Diagnostics                                                               */
+/* ------------------------------------------------------------------------ */
+
+/* No counterpart in the reference.  Evaluates on the device, for n operand
+ * pairs (a[i], b[i]), the shortened division the stage kernels use for the
+ * WENO weight divisions when max|u| and eps allow it, and the IEEE division
+ * '/' compiles to: mode 0 gives a/b, mode 1 and 2 the WENO-Z weight
+ * 0.3*(1 + (a/b)^p) with p = mode.  q_fast and q_ieee receive the two
+ * results, for a bitwise comparison by the caller. */
+int mgad_check_division(hipStream_t stream, int mode, const double *a,
+                        const double *b, double *q_fast, double *q_ieee,
+                        size_t n);
+
+/* ------------------------------------------------------------------------ */
 /* Host helpers (CPU only)                                                   */
 /* ------------------------------------------------------------------------ */
 

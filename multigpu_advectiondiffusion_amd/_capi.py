@@ -27,7 +27,7 @@ EXPORTS = (
     "mgad_burgers_init_p", "mgad_burgers_steps_p", "mgad_burgers_autotune", "mgad_burgers_set_launch",
     "mgad_batch_workspace_bytes", "mgad_batch_init", "mgad_batch_steps", "mgad_batch_steps_serial",
     "mgad_adjoint_workspace_bytes", "mgad_burgers_adjoint",
-    "mgad_heat7_fe_steps", "mgad_ic_fourier", "mgad_ic_gauss_noise",
+    "mgad_heat7_fe_steps", "mgad_check_division", "mgad_ic_fourier", "mgad_ic_gauss_noise",
 )
 
 
@@ -131,6 +131,8 @@ def lib():
         L.mgad_burgers_read_ctrl.restype = I
         L.mgad_heat7_fe_steps.argtypes = [VP, U, U, U, D, D, D, D, VP, VP, I, I]
         L.mgad_heat7_fe_steps.restype = I
+        L.mgad_check_division.argtypes = [VP, I, VP, VP, VP, VP, ctypes.c_size_t]
+        L.mgad_check_division.restype = I
         L.mgad_ic_fourier.argtypes = [VP, U, U, U, D, D, ctypes.c_uint64, I, I, D, D]
         L.mgad_ic_fourier.restype = I
         L.mgad_ic_gauss_noise.argtypes = [VP, U, U, U, D, D, ctypes.c_uint64]
@@ -144,7 +146,10 @@ def error_string(code: int) -> str:
 
 
 def check(rc: int, what: str) -> None:
+    """Raise on a failed status; the failure is delivered, so the library's
+    sticky copy of it is read and cleared too."""
     if rc != MGAD_OK:
+        lib().mgad_get_last_error()
         raise MgadError(f"{what}: {error_string(rc)} (code {rc})")
 
 

@@ -47,8 +47,12 @@ __global__ __launch_bounds__(kBT *kBT, 4) void k_burgers_batch(const BatchProble
         const T *u = static_cast<const T *>(bp.u);
         const T *uin = stage == 1 ? u : (stage == 2 ? static_cast<const T *>(bp.w1) : static_cast<const T *>(bp.w2));
         T *uout = stage == 1 ? static_cast<T *>(bp.w1) : (stage == 2 ? static_cast<T *>(bp.w2) : static_cast<T *>(bp.u));
-        stage_tile<T, WENO, P, kBT, kBT>(uin, u, uout, bp.g, stage, tile, dt, alpha, eps, sm,
-                                         &bp.ctrl->amax[stage % 3]);
+        if (fast_division_ok<WENO>(alpha, eps))
+            stage_tile<T, WENO, P, kBT, kBT, true>(uin, u, uout, bp.g, stage, tile, dt, alpha, eps, sm,
+                                                   &bp.ctrl->amax[stage % 3]);
+        else
+            stage_tile<T, WENO, P, kBT, kBT, false>(uin, u, uout, bp.g, stage, tile, dt, alpha, eps, sm,
+                                                    &bp.ctrl->amax[stage % 3]);
     }
 }
 

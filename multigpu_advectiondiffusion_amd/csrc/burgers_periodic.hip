@@ -43,8 +43,12 @@ __global__ __launch_bounds__(TX * TY, 4) void k_burgers_stage(const T *__restric
     const bool lead = (blockIdx.x == 0 && threadIdx.x == 0);
     if (!stage_control(ctrl, stage, g.dx, lead, dt, alpha)) return;
     const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
-    stage_tile<T, WENO, P, TX, TY>(uin, un, uout, g, stage, tile, dt, alpha, eps, sm,
-                                   &ctrl->amax[stage % 3]);
+    if (fast_division_ok<WENO>(alpha, eps))
+        stage_tile<T, WENO, P, TX, TY, true>(uin, un, uout, g, stage, tile, dt, alpha, eps, sm,
+                                             &ctrl->amax[stage % 3]);
+    else
+        stage_tile<T, WENO, P, TX, TY, false>(uin, un, uout, g, stage, tile, dt, alpha, eps, sm,
+                                              &ctrl->amax[stage % 3]);
 }
 
 // max|u| over n values into *slot (atomicMax of the bit pattern).

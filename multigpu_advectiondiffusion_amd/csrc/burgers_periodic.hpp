@@ -87,8 +87,8 @@ struct TileSmem {
 };
 
 // One tile x z-chunk of one stage; ends with one atomicMax per block into the
-// stage's max|u| slot.
-template <class T, int WENO, int P, int TX, int TY>
+// stage's max|u| slot.  FD: fast_division_ok<WENO>(alpha, eps) holds.
+template <class T, int WENO, int P, int TX, int TY, bool FD>
 __device__ __forceinline__ void stage_tile(const T *__restrict__ uin, const T *un, T *uout,
                                            const GridArgs &g, int stage, unsigned tile, double dt,
                                            double alpha, double eps, TileSmem<TX, TY> &sm,
@@ -146,7 +146,7 @@ __device__ __forceinline__ void stage_tile(const T *__restrict__ uin, const T *u
         sn[q] = sterm(qm[q + 1], qm[q + 2], qm[q + 3]);
     }
     // face z0-1/2: f+ at z0-3..z0+1 (centres z0-2..z0), f- at z0-2..z0+2 (centres z0-1..z0+1)
-    double fz_prev = weno_face_s<WENO, P>(qp[0], qp[1], qp[2], qp[3], qp[4], qm[1], qm[2], qm[3],
+    double fz_prev = weno_face_s<WENO, P, FD>(qp[0], qp[1], qp[2], qp[3], qp[4], qm[1], qm[2], qm[3],
                                           qm[4], qm[5], sp0, sp[0], sp[1], sn[0], sn[1], sn[2], eps);
 
     const T *own_ptr = uin + col;
@@ -180,19 +180,19 @@ __device__ __forceinline__ void stage_tile(const T *__restrict__ uin, const T *u
         __syncthreads();
 
         // z face k+1/2 from the register queue; x face x+1/2 and y face y+1/2 from LDS
-        const double fz = weno_face_s<WENO, P>(qp[1], qp[2], qp[3], qp[4], qp[5], qm[2], qm[3], qm[4],
+        const double fz = weno_face_s<WENO, P, FD>(qp[1], qp[2], qp[3], qp[4], qp[5], qm[2], qm[3], qm[4],
                                                qm[5], qm[6], sp[0], sp[1], sp[2], sn[1], sn[2], sn[3],
                                                eps);
         {
             const double2 a0 = sm.P[ty + 3][tx + 1], a1 = sm.P[ty + 3][tx + 2], a2 = sm.P[ty + 3][tx + 3],
                           a3 = sm.P[ty + 3][tx + 4], a4 = sm.P[ty + 3][tx + 5], a5 = sm.P[ty + 3][tx + 6];
-            sm.FX[ty][tx + 1] = weno_face2<WENO, P>(a0.x, a1.x, a2.x, a3.x, a4.x, a1.y, a2.y, a3.y,
+            sm.FX[ty][tx + 1] = weno_face2<WENO, P, FD>(a0.x, a1.x, a2.x, a3.x, a4.x, a1.y, a2.y, a3.y,
                                                     a4.y, a5.y, eps);
         }
         {
             const double2 a0 = sm.P[ty + 1][tx + 3], a1 = sm.P[ty + 2][tx + 3], a2 = sm.P[ty + 3][tx + 3],
                           a3 = sm.P[ty + 4][tx + 3], a4 = sm.P[ty + 5][tx + 3], a5 = sm.P[ty + 6][tx + 3];
-            sm.FY[ty + 1][tx] = weno_face2<WENO, P>(a0.x, a1.x, a2.x, a3.x, a4.x, a1.y, a2.y, a3.y,
+            sm.FY[ty + 1][tx] = weno_face2<WENO, P, FD>(a0.x, a1.x, a2.x, a3.x, a4.x, a1.y, a2.y, a3.y,
                                                     a4.y, a5.y, eps);
         }
         // faces on the tile's low edges, by one wave that rotates with k
@@ -207,7 +207,7 @@ __device__ __forceinline__ void stage_tile(const T *__restrict__ uin, const T *u
                 a0 = sm.P[0][c]; a1 = sm.P[1][c]; a2 = sm.P[2][c];
                 a3 = sm.P[3][c]; a4 = sm.P[4][c]; a5 = sm.P[5][c];
             }
-            const double f = weno_face2<WENO, P>(a0.x, a1.x, a2.x, a3.x, a4.x, a1.y, a2.y, a3.y, a4.y,
+            const double f = weno_face2<WENO, P, FD>(a0.x, a1.x, a2.x, a3.x, a4.x, a1.y, a2.y, a3.y, a4.y,
                                                  a5.y, eps);
             if (lane < TY)
                 sm.FX[lane][0] = f;
@@ -226,7 +226,7 @@ __device__ __forceinline__ void stage_tile(const T *__restrict__ uin, const T *u
         else if (stage == 2)
             unew = 0.75 * uo + 0.25 * (qu[0] + dt * Lu);
         else
-            unew = div_by(uo + 2.0 * (qu[0] + dt * Lu), 3.0, 1.0 / 3.0);
+            unew = div_by(p2fma(2.0, qu[0] + dt * Lu, uo), 3.0, 1.0 / 3.0);  // (uo + 2(..)) / 3
         const T stored = static_cast<T>(unew);
         out_ptr[plane * static_cast<unsigned>(k)] = stored;
         const unsigned long long b = abs_bits(static_cast<double>(stored));

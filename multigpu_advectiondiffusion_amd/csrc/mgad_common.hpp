@@ -48,37 +48,85 @@ __device__ __forceinline__ double div_by(double a, double b, double rb) {
     return __builtin_fma(rem, rb, q);
 }
 
+// k*x + y for k = +-2 or +-4, one instruction.  k*x is exact in binary64
+// (a power-of-two scaling; the only exception is overflow, |x| >= 2^1021), so
+// the single rounding of the fused form is the same double as the two-step
+// RN(RN(k*x) + y) the source expressions spell out.  A subtraction a - k*b is
+// written p2fma(-k, b, a): IEEE defines a - y as a + (-y).
+__device__ __forceinline__ double p2fma(double k, double x, double y) { return __builtin_fma(k, x, y); }
+
 // First term of a smoothness indicator, 13/12 (a-2b+c)^2 (kernels.cu:157-159):
 // it depends only on the sub-stencil's centre, so callers that march along a
 // line compute it once per node and pass it to weno_side_s.
 __device__ __forceinline__ double sterm(double a, double b, double c) {
-    const double s = (a - 2.0 * b) + c;
+    const double s = p2fma(-2.0, b, a) + c;  // (a - 2b) + c
     return (kC1312 * s) * s;
 }
 
 // Jiang-Shu smoothness indicators of the three 3-point sub-stencils
 // (kernels.cu:157-159): 13/12 (a-2b+c)^2 + 1/4 (a-4b+3c)^2, etc.
 __device__ __forceinline__ double smooth0(double a, double b, double c) {
-    const double s = (a - 2.0 * b) + c;
-    const double q = (a - 4.0 * b) + 3.0 * c;
+    const double s = p2fma(-2.0, b, a) + c;         // (a - 2b) + c
+    const double q = p2fma(-4.0, b, a) + 3.0 * c;   // (a - 4b) + 3c
     return (kC1312 * s) * s + (kC14 * q) * q;
 }
 __device__ __forceinline__ double smooth1(double b, double c, double d) {
-    const double s = (b - 2.0 * c) + d;
+    const double s = p2fma(-2.0, c, b) + d;         // (b - 2c) + d
     const double q = b - d;
     return (kC1312 * s) * s + (kC14 * q) * q;
 }
 __device__ __forceinline__ double smooth2(double c, double d, double e) {
-    const double s = (c - 2.0 * d) + e;
-    const double q = (3.0 * c - 4.0 * d) + e;
+    const double s = p2fma(-2.0, d, c) + e;         // (c - 2d) + e
+    const double q = p2fma(-4.0, d, 3.0 * c) + e;   // (3c - 4d) + e
     return (kC1312 * s) * s + (kC14 * q) * q;
+}
+
+// a/b by the reciprocal iteration that IEEE division compiles to on gfx950
+// (v_rcp_f64, two Newton steps, q = a*r, one residual correction) without its
+// operand scaling (v_div_scale), scaled last FMA (v_div_fmas) and special-case
+// fix-up (v_div_fixup).  Those three return their inputs unchanged when
+//   0 < b, 2^-1000 <= b <= 2^1000,
+//   a = +0, or 2^-960 <= |a| <= 2^1000,
+//   |exponent(a) - exponent(b)| <= 760,
+// (no huge exponent gap, no subnormal reciprocal or quotient, no tiny
+// numerator), so there the result is the same double as a / b
+// (mgad_check_division compares the two on the device).
+__device__ __forceinline__ double div_inrange(double a, double b) {
+    double r = __builtin_amdgcn_rcp(b);
+    double e = __builtin_fma(-b, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    e = __builtin_fma(-b, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    const double q = a * r;
+    const double rem = __builtin_fma(-b, q, a);
+    return __builtin_fma(rem, r, q);
+}
+
+// When the weight divisions of weno_side_s may use div_inrange (its FD
+// variant) with results identical to IEEE division.  With max|u| <= 2^40 over
+// the stage input every split flux has |f| < 2^80, so each 13/12 s^2 < 2^165
+// and 1/4 q^2 < 2^164: 0 <= beta_k < 2^166.
+//   WENO-Z, 2^-300 <= eps <= 1: b = beta+eps is in [2^-300, 2^167] and
+//     a = tau5 < 2^167.  For tau5 = 0 or tau5 >= 2^-380 the ranges above
+//     hold.  For 0 < tau5 < 2^-380 both quotients are below 2^-79, so
+//     1 + q (p=1) and 1 + q^2 (p=2) round to exactly 1 either way and the
+//     weight is d_k in both.
+//   WENO-JS, 2^-190 <= eps <= 1: (eps+beta)^2 is in [2^-380, 2^334], so
+//     alpha_k = d_k/(..)^2 is in [2^-338, 2^380] and alpha_k/sum in
+//     [2^-720, 1].
+// The final division of WENO-Z (numerator unbounded below) stays IEEE.  A NaN
+// or infinite alpha fails the test.
+template <int WENO>
+__host__ __device__ inline bool fast_division_ok(double alpha, double eps) {
+    const double eps_lo = WENO == kWenoZ ? 0x1p-300 : 0x1p-190;
+    return alpha <= 0x1p40 && eps >= eps_lo && eps <= 1.0;
 }
 
 // WENO-Z weight d*(1 + (tau5/(beta+eps))^P), kernels.cu:163-165 (P = 1);
 // P = 2 is the Borges et al. (2008) choice.
-template <int P>
+template <int P, bool FD = false>
 __device__ __forceinline__ double zweight(double d, double tau5, double beta, double eps) {
-    const double q = tau5 / (beta + eps);
+    const double q = FD ? div_inrange(tau5, beta + eps) : tau5 / (beta + eps);
     return d * (1.0 + (P == 2 ? q * q : q));
 }
 
@@ -86,65 +134,74 @@ __device__ __forceinline__ double zweight(double d, double tau5, double beta, do
 // PLUS: left-biased f+ side, v = f+ at i-2..i+2, weights (1,6,3)/10;
 // else the right-biased f- side, v = f- at i-1..i+3, weights (3,6,1)/10.
 // B_k = S_k + 1/4 q_k^2 is the reference's sum in its own order.
-template <int WENO, int P, bool PLUS>
+// FD: fast_division_ok holds for the stage, and the weight divisions use
+// div_inrange.
+template <int WENO, int P, bool PLUS, bool FD = false>
 __device__ __forceinline__ double weno_side_s(double v0, double v1, double v2, double v3, double v4,
                                               double S0, double S1, double S2, double eps) {
-    const double q0 = (v0 - 4.0 * v1) + 3.0 * v2;
+    // the source expressions are in the comments; p2fma folds the exact
+    // products by 2 and 4 into the adjacent addition
+    const double q0 = p2fma(-4.0, v1, v0) + 3.0 * v2;  // (v0 - 4v1) + 3v2
     const double q1 = v1 - v3;
-    const double q2 = (3.0 * v2 - 4.0 * v3) + v4;
+    const double q2 = p2fma(-4.0, v3, 3.0 * v2) + v4;  // (3v2 - 4v3) + v4
     const double B0 = S0 + (kC14 * q0) * q0;
     const double B1 = S1 + (kC14 * q1) * q1;
     const double B2 = S2 + (kC14 * q2) * q2;
     double c0, c1, c2, d0, d1, d2;
     if (PLUS) {
-        c0 = (2.0 * v0 - 7.0 * v1) + 11.0 * v2;
-        c1 = (-v1 + 5.0 * v2) + 2.0 * v3;
-        c2 = (2.0 * v2 + 5.0 * v3) - v4;
+        c0 = p2fma(2.0, v0, -(7.0 * v1)) + 11.0 * v2;  // (2v0 - 7v1) + 11v2
+        c1 = p2fma(2.0, v3, -v1 + 5.0 * v2);           // (-v1 + 5v2) + 2v3
+        c2 = p2fma(2.0, v2, 5.0 * v3) - v4;            // (2v2 + 5v3) - v4
         d0 = kD0N; d1 = kD1N; d2 = kD2N;
     } else {
-        c0 = (-v0 + 5.0 * v1) + 2.0 * v2;
-        c1 = (2.0 * v1 + 5.0 * v2) - v3;
-        c2 = (11.0 * v2 - 7.0 * v3) + 2.0 * v4;
+        c0 = p2fma(2.0, v2, -v0 + 5.0 * v1);           // (-v0 + 5v1) + 2v2
+        c1 = p2fma(2.0, v1, 5.0 * v2) - v3;            // (2v1 + 5v2) - v3
+        c2 = p2fma(2.0, v4, 11.0 * v2 - 7.0 * v3);     // (11v2 - 7v3) + 2v4
         d0 = kD0P; d1 = kD1P; d2 = kD2P;
     }
     if (WENO == kWenoZ) {
         const double tau5 = fabs(B0 - B2);
-        const double a0 = zweight<P>(d0, tau5, B0, eps);
-        const double a1 = zweight<P>(d1, tau5, B1, eps);
-        const double a2 = zweight<P>(d2, tau5, B2, eps);
+        const double a0 = zweight<P, FD>(d0, tau5, B0, eps);
+        const double a1 = zweight<P, FD>(d1, tau5, B1, eps);
+        const double a2 = zweight<P, FD>(d2, tau5, B2, eps);
         const double sum = (a0 + a1) + a2;
         return ((a0 * c0 + a1 * c1) + a2 * c2) / (6.0 * sum);
     } else {
-        const double a0 = d0 / ((eps + B0) * (eps + B0));
-        const double a1 = d1 / ((eps + B1) * (eps + B1));
-        const double a2 = d2 / ((eps + B2) * (eps + B2));
+        const double e0 = (eps + B0) * (eps + B0);
+        const double e1 = (eps + B1) * (eps + B1);
+        const double e2 = (eps + B2) * (eps + B2);
+        const double a0 = FD ? div_inrange(d0, e0) : d0 / e0;
+        const double a1 = FD ? div_inrange(d1, e1) : d1 / e1;
+        const double a2 = FD ? div_inrange(d2, e2) : d2 / e2;
         const double sum = (a0 + a1) + a2;
-        const double o0 = a0 / sum, o1 = a1 / sum, o2 = a2 / sum;
-        return ((o0 * c0 + o1 * c1) + o2 * c2) / 6.0;
+        const double o0 = FD ? div_inrange(a0, sum) : a0 / sum;
+        const double o1 = FD ? div_inrange(a1, sum) : a1 / sum;
+        const double o2 = FD ? div_inrange(a2, sum) : a2 / sum;
+        return div_by((o0 * c0 + o1 * c1) + o2 * c2, 6.0, 1.0 / 6.0);
     }
 }
 
 // Face flux from split fluxes with the 13/12 terms precomputed:
 // Sp_k at the centres of the f+ sub-stencils (i-1, i, i+1), Sm_k at the
 // centres of the f- sub-stencils (i, i+1, i+2).
-template <int WENO, int P>
+template <int WENO, int P, bool FD = false>
 __device__ __forceinline__ double weno_face_s(double v0, double v1, double v2, double v3, double v4,
                                               double w0, double w1, double w2, double w3, double w4,
                                               double Sp0, double Sp1, double Sp2, double Sm0,
                                               double Sm1, double Sm2, double eps) {
-    const double hp = weno_side_s<WENO, P, true>(v0, v1, v2, v3, v4, Sp0, Sp1, Sp2, eps);
-    const double hm = weno_side_s<WENO, P, false>(w0, w1, w2, w3, w4, Sm0, Sm1, Sm2, eps);
+    const double hp = weno_side_s<WENO, P, true, FD>(v0, v1, v2, v3, v4, Sp0, Sp1, Sp2, eps);
+    const double hm = weno_side_s<WENO, P, false, FD>(w0, w1, w2, w3, w4, Sm0, Sm1, Sm2, eps);
     return hp + hm;
 }
 
 // Reconstructed flux at face i+1/2:  v = f+ at i-2..i+2, w = f- at i-1..i+3.
 // WENO-Z: kernels.cu:142-192 (alpha / (6*sum)).
 // WENO-JS: MultiGPU/Burgers3d_Baseline/Kernels.cu:112-220 (w = alpha/sum, /6).
-template <int WENO, int P>
+template <int WENO, int P, bool FD = false>
 __device__ __forceinline__ double weno_face2(double v0, double v1, double v2, double v3, double v4,
                                              double w0, double w1, double w2, double w3, double w4,
                                              double eps) {
-    return weno_face_s<WENO, P>(v0, v1, v2, v3, v4, w0, w1, w2, w3, w4, sterm(v0, v1, v2),
+    return weno_face_s<WENO, P, FD>(v0, v1, v2, v3, v4, w0, w1, w2, w3, w4, sterm(v0, v1, v2),
                                 sterm(v1, v2, v3), sterm(v2, v3, v4), sterm(w0, w1, w2),
                                 sterm(w1, w2, w3), sterm(w2, w3, w4), eps);
 }
@@ -157,9 +214,9 @@ __device__ __forceinline__ double weno_face(double v0, double v1, double v2, dou
     double hp, hm;
     {
         const double B0 = smooth0(v0, v1, v2), B1 = smooth1(v1, v2, v3), B2 = smooth2(v2, v3, v4);
-        const double p0 = (2.0 * v0 - 7.0 * v1) + 11.0 * v2;
-        const double p1 = (-v1 + 5.0 * v2) + 2.0 * v3;
-        const double p2 = (2.0 * v2 + 5.0 * v3) - v4;
+        const double p0 = p2fma(2.0, v0, -(7.0 * v1)) + 11.0 * v2;
+        const double p1 = p2fma(2.0, v3, -v1 + 5.0 * v2);
+        const double p2 = p2fma(2.0, v2, 5.0 * v3) - v4;
         if (WENO == kWenoZ) {
             const double tau5 = fabs(B0 - B2);
             const double a0 = zweight<P>(kD0N, tau5, B0, eps);
@@ -178,9 +235,9 @@ __device__ __forceinline__ double weno_face(double v0, double v1, double v2, dou
     }
     {
         const double B0 = smooth0(w0, w1, w2), B1 = smooth1(w1, w2, w3), B2 = smooth2(w2, w3, w4);
-        const double m0 = (-w0 + 5.0 * w1) + 2.0 * w2;
-        const double m1 = (2.0 * w1 + 5.0 * w2) - w3;
-        const double m2 = (11.0 * w2 - 7.0 * w3) + 2.0 * w4;
+        const double m0 = p2fma(2.0, w2, -w0 + 5.0 * w1);
+        const double m1 = p2fma(2.0, w1, 5.0 * w2) - w3;
+        const double m2 = p2fma(2.0, w4, 11.0 * w2 - 7.0 * w3);
         if (WENO == kWenoZ) {
             const double tau5 = fabs(B0 - B2);
             const double a0 = zweight<P>(kD0P, tau5, B0, eps);
@@ -219,7 +276,7 @@ __device__ __forceinline__ void lf_split_local(double u, double &fp, double &fm)
 __device__ __forceinline__ double rk_stage(int step, double dt, double u, double uo, double Lu) {
     if (step == 1) return uo + dt * Lu;
     if (step == 2) return 0.75 * uo + 0.25 * (u + dt * Lu);
-    return (uo + 2.0 * (u + dt * Lu)) / 3.0;
+    return p2fma(2.0, u + dt * Lu, uo) / 3.0;  // (uo + 2(u + dt Lu)) / 3
 }
 
 // Non-negative doubles order like their bit patterns as unsigned integers, so

@@ -25,6 +25,7 @@ def field(shape, seed, lo=-1.0, hi=2.0):
 @pytest.fixture(autouse=True)
 def reset_scheme():
     R.set_adv_scheme()
+    assert _capi.last_error() == 0, "an earlier call left an unread error"
     yield
     R.set_adv_scheme()
 
