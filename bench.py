@@ -174,6 +174,8 @@ def main() -> int:
         print("batched study done", file=sys.stderr, flush=True)
         a = studies.adjoint(n=128)
         out["adjoint"] = {"cost_ratio": a["cost_ratio"], "steps": a["cost_steps"], "grid": 128,
+                          "checkpoint_every": a["sqrtN"]["checkpoint_every"],
+                          "cost_ratio_store_all": a["store_all"]["cost_ratio"],
                           "fd_max_err_over_grad_norm_h1e-6": max(a["T0.2"]["h1e-06"]["max_err_over_grad_norm"],
                                                                  a["T0.5"]["h1e-06"]["max_err_over_grad_norm"]),
                           "fd_median_rel_err_h1e-6": max(a["T0.2"]["h1e-06"]["median_rel_err"],

@@ -148,24 +148,34 @@ def adjoint(n: int = 128, steps: int = 400, ndir: int = 16, h: float = 1e-6) -> 
     s.steps(steps)
     T = s.state().t
     target, _ = forward(u0 + 1e-2 * noise, T)
-    adj = solvers.BurgersAdjoint(n, scheme="Z2", max_steps=steps + 8, stream=s.stream)
-    adj(u0, target, T)  # warm-up (graphs, caches)
     e0, e1 = hip.Event(), hip.Event()
-    hip.synchronize()
-    e0.record(s.stream)
-    s.set_initial(u0, tend=T)
-    st = s.run(batch=steps + 8)
-    e1.record(s.stream)
-    e1.synchronize()
-    t_fwd = e0.elapsed_ms(e1) / 1e3
-    hip.synchronize()
-    e0.record(s.stream)
-    J, g, nst = adj(u0, target, T)
-    e1.record(s.stream)
-    e1.synchronize()
-    t_adj = e0.elapsed_ms(e1) / 1e3
-    out.update({"cost_steps": nst, "cost_T": T, "forward_s": t_fwd, "adjoint_s": t_adj,
-                "cost_ratio": t_adj / t_fwd, "checkpoint_every": adj.every})
+
+    def timed(fn):
+        hip.synchronize()
+        e0.record(s.stream)
+        r = fn()
+        e1.record(s.stream)
+        e1.synchronize()
+        return r, e0.elapsed_ms(e1) / 1e3
+
+    def fwd():
+        s.set_initial(u0, tend=T)
+        return s.run(batch=steps + 8)
+
+    fwd()  # warm-up
+    _, t_fwd = timed(fwd)
+    out.update({"cost_T": T, "forward_s": t_fwd})
+    # sqrt(N) checkpoints (the default), then every stage input kept (no recompute)
+    for name, every in (("sqrtN", None), ("store_all", steps + 8)):
+        adj = solvers.BurgersAdjoint(n, scheme="Z2", max_steps=steps + 8, every=every, stream=s.stream)
+        adj(u0, target, T)  # warm-up
+        (J, g, nst), t_adj = timed(lambda: adj(u0, target, T))
+        out[name] = {"checkpoint_every": adj.every, "adjoint_s": t_adj, "cost_ratio": t_adj / t_fwd,
+                     "workspace_gb": adj.workspace.nbytes / 1e9, "steps": nst}
+        del adj
+    out["cost_steps"] = out["sqrtN"]["steps"]
+    out["cost_ratio"] = out["sqrtN"]["cost_ratio"]
+    adj = solvers.BurgersAdjoint(n, scheme="Z2", max_steps=steps + 8, stream=s.stream)
     rng = np.random.default_rng(2026)
     for tend in (0.2, 0.5):
         tg, _ = forward(u0 + 1e-2 * noise, tend)
