@@ -109,11 +109,6 @@ __device__ __forceinline__ double side_grad(double v0, double v1, double v2, dou
     return h;
 }
 
-// Per-step record written during the forward recompute and read in reverse.
-struct StepRec {
-    double dt, a1, a2, a3, dt_cfl;
-};
-
 // Device-side scalars of the reverse sweep.
 struct AdjScalars {
     double dtb;                // accumulated dJ/d(dt) of the current step
@@ -123,57 +118,96 @@ struct AdjScalars {
     unsigned pad;
 };
 
-// Records dt, alpha of stage 1 and the unclipped CFL step cfl*dx/a1 after
-// stage 1 (a clipped last step is recognised by dt != dt_cfl), the alphas of
-// stages 2 and 3 after stage 2.  A step past tend records dt = 0 (unused).
-__global__ void k_record(const Ctrl *c, StepRec *rec, int after_stage, double cfl_dx) {
-    if (after_stage == 1) {
-        rec->dt = c->dt;
-        rec->a1 = bits_double(c->amax[0]);
-        rec->dt_cfl = cfl_dx / rec->a1;
-    } else {
-        rec->a2 = bits_double(c->amax[1]);
-        rec->a3 = bits_double(c->amax[2]);
-    }
+// Points the run's step record at rec, starting with the current step.
+__global__ void k_ctrl_rec(Ctrl *c, StepRec *rec) {
+    c->rec = rec;
+    c->rec_step0 = c->steps;
 }
 
 // Contributions of one face to the six nodes of its stencil: the face
-// i+1/2 reads f+ at nodes i-2..i+2 (v) and f- at i-1..i+3 (w); node t of
-// i-2..i+3 receives Fb * (dF/df+_t * (u_t+alpha)/2 + dF/df-_t * (u_t-alpha)/2).
+// i+1/2 reads f+ at nodes i-2..i+2 and f- at i-1..i+3; node t of i-2..i+3
+// receives Fb * (dF/df+_t * (u_t+alpha)/2 + dF/df-_t * (u_t-alpha)/2).
 // Accumulates F*Fb (the face's share of <lam, L>) and Fb * sum_t u_t/2 *
-// (dF/df+_t - dF/df-_t) (its share of <lam, dL/dalpha>).
-template <int WENO, int P>
+// (dF/df+_t - dF/df-_t) (its share of <lam, dL/dalpha>).  SEQ: the two sides
+// are evaluated one after the other (a scheduling barrier keeps the compiler
+// from interleaving them), which lowers the live registers.
+template <int WENO, int P, bool SEQ>
 __device__ __forceinline__ void face_vjp(const double (&fp)[6], const double (&fm)[6], const double (&uu)[6],
                                          double Fb, double alpha, double eps, double (&c)[6], double &sL,
                                          double &sLa) {
-    double gp[5], gm[5];
-    const double F = side_grad<WENO, P, true>(fp[0], fp[1], fp[2], fp[3], fp[4], eps, gp) +
-                     side_grad<WENO, P, false>(fm[1], fm[2], fm[3], fm[4], fm[5], eps, gm);
-    sL += F * Fb;
+    double g[5];
+    const double hp = side_grad<WENO, P, true>(fp[0], fp[1], fp[2], fp[3], fp[4], eps, g);
     double la = 0.0;
 #pragma unroll
-    for (int t = 0; t < 6; t++) {
-        const double Gp = t < 5 ? Fb * gp[t] : 0.0;
-        const double Gm = t > 0 ? Fb * gm[t - 1] : 0.0;
-        c[t] = Gp * (0.5 * (uu[t] + alpha)) + Gm * (0.5 * (uu[t] - alpha));
-        la += 0.5 * uu[t] * (Gp - Gm);
+    for (int t = 0; t < 5; t++) {
+        const double G = Fb * g[t];
+        c[t] = G * (0.5 * (uu[t] + alpha));
+        la += 0.5 * uu[t] * G;
     }
+    c[5] = 0.0;
+    if (SEQ) __builtin_amdgcn_sched_barrier(0);
+    const double hm = side_grad<WENO, P, false>(fm[1], fm[2], fm[3], fm[4], fm[5], eps, g);
+#pragma unroll
+    for (int t = 0; t < 5; t++) {
+        const double G = Fb * g[t];
+        c[t + 1] += G * (0.5 * (uu[t + 1] - alpha));
+        la -= 0.5 * uu[t + 1] * G;
+    }
+    sL += (hp + hm) * Fb;
     sLa += la;
 }
 
-// Grid of the VJP launch: extents, spacings, planes per z chunk (the last
-// chunk also takes the remainder, so every chunk has at least 5 planes).
+// face_vjp for a face whose stencil lives in LDS: split fluxes Pp[t*st],
+// Pm[t*st] and u U[t*st] of nodes t = 0..5 (i-2..i+3), contributions written
+// to G[t*gst].  Each side loads only its own inputs, after the previous side
+// is done, so fewer values are live at once.
+template <int WENO, int P>
+__device__ __forceinline__ void face_vjp_lds(const double *Pp, const double *Pm, const double *Up, int st, double Fb,
+                                             double alpha, double eps, double *G, int gst, double &sL, double &sLa) {
+    double g[5], c[5];
+    double la = 0.0;
+    const double hp = side_grad<WENO, P, true>(Pp[0], Pp[st], Pp[2 * st], Pp[3 * st], Pp[4 * st], eps, g);
+#pragma unroll
+    for (int t = 0; t < 5; t++) {
+        const double ut = Up[t * st];
+        const double Gt = Fb * g[t];
+        c[t] = Gt * (0.5 * (ut + alpha));
+        la += 0.5 * ut * Gt;
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    const double hm = side_grad<WENO, P, false>(Pm[st], Pm[2 * st], Pm[3 * st], Pm[4 * st], Pm[5 * st], eps, g);
+    G[0] = c[0];
+#pragma unroll
+    for (int t = 0; t < 5; t++) {
+        const double ut = Up[(t + 1) * st];
+        const double Gt = Fb * g[t];
+        const double m = Gt * (0.5 * (ut - alpha));
+        G[(t + 1) * gst] = t < 4 ? c[t + 1] + m : m;
+        la -= 0.5 * ut * Gt;
+    }
+    sL += (hp + hm) * Fb;
+    sLa += la;
+}
+
+// Grid of the VJP launches: extents, spacings, the number of plane ranges of
+// the x/y blocks (nxy) and of the z march (nzc).  Range c of n covers planes
+// [c*nz/n, (c+1)*nz/n); z ranges have at least 5 planes.
 struct VjpGrid {
-    int nx, ny, nz, kchunk, nchunks;
+    int nx, ny, nz, nxy, nzc;
     double dx, dy, dz;
 };
 
-// LDS of one VJP tile: split fluxes and u of the current plane with a halo of
-// two nodes below and three above the tile in x and y, lambda with one node
+__host__ __device__ inline int range_lo(int c, int n, int nz) {
+    return static_cast<int>(static_cast<long long>(c) * nz / n);
+}
+
+// LDS of one x/y VJP tile: split fluxes and u of the current plane with a halo
+// of two nodes below and three above the tile in x and y, lambda with one node
 // above, and the six node contributions of every x and y face of the tile.
 template <int TX, int TY>
 struct VjpSmem {
-    double2 P[TY + 5][TX + 5];
+    double Pp[TY + 5][TX + 5];  // f+
+    double Pm[TY + 5][TX + 5];  // f-
     double U[TY + 5][TX + 5];
     double Lm[TY + 1][TX + 1];
     double Gx[6][TY][TX];  // Gx[t][ty][tx]: face x0+tx+1/2 to node x0+tx-2+t
@@ -181,24 +215,47 @@ struct VjpSmem {
     double red[2][TX * TY / 64];
 };
 
-// Stage VJP of the periodic RHS, all three axes in one pass: ubi = pass-through
-// + sum over the faces of L of Fb * dF/du, with Fb = (lam_{i+1} - lam_i)/d and
-// lam = coef * ubo.  A block owns a TX x TY column tile and a z chunk
-// [z0, z1); it computes each face it owns once -- x faces x0..x0+TX-1 (+1/2),
-// y faces likewise, z faces z0..z1-1 -- one of each per thread and plane.
-// Contributions to nodes of the tile are gathered in a fixed order; those to
-// nodes of a neighbouring tile or chunk (two below, three above in each axis)
-// go to the exchange fields exx/exy/exz, which k_vjp_inflow adds afterwards.
-// The z march finishes node k-2 once its last own z face, k+1/2, is done.
-// Pass-through terms: stage 3 ubi = 2/3 ubo, acc = 1/3 ubo; stage 2
-// ubi = 1/4 ubo, acc += 3/4 ubo; stage 1 ubi (= acc, in place) = acc + ubo.
+// Block partial sums of <lam, L> and <lam, dL/dalpha> (fixed order).
+template <int NT>
+__device__ __forceinline__ void block_partials(double sL, double sLa, double (&red)[2][NT / 64], double *out) {
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        sL += __shfl_xor(sL, off, 64);
+        sLa += __shfl_xor(sLa, off, 64);
+    }
+    if (lane == 0) {
+        red[0][wave] = sL;
+        red[1][wave] = sLa;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        double a = 0.0, b = 0.0;
+        for (int w = 0; w < NT / 64; w++) {
+            a += red[0][w];
+            b += red[1][w];
+        }
+        out[2 * blockIdx.x] = a;
+        out[2 * blockIdx.x + 1] = b;
+    }
+}
+
+// x and y part of the stage VJP of the periodic RHS: for every node,
+// ubi = pass-through + the contributions of the x and y faces of its tile,
+// with Fb = (lam_{i+1} - lam_i)/d and lam = coef * ubo.  A block owns a
+// TX x TY column tile over a range of planes and computes the tile's own
+// faces x0..x0+TX-1 (+1/2) and y0..y0+TY-1 (+1/2) once each, one x and one y
+// face per thread and plane.  Contributions to nodes of a neighbouring tile
+// (two below, three above in each axis) go to the exchange fields exx / exy;
+// k_vjp_z adds them.  Pass-through terms: stage 3 ubi = 2/3 ubo,
+// acc = 1/3 ubo; stage 2 ubi = 1/4 ubo, acc += 3/4 ubo; stage 1 ubi (= acc,
+// in place) = acc + ubo.
 template <int WENO, int P, int TX, int TY>
-__global__ __launch_bounds__(TX * TY, 2) void k_vjp_tile(
+__global__ __launch_bounds__(TX * TY, 3) void k_vjp_xy(
     const double *__restrict__ u, const double *__restrict__ ubo, double *ubi, double *acc, const StepRec *rec,
-    int stage, VjpGrid g, double eps, double *__restrict__ exx, double *__restrict__ exy, double *__restrict__ exz,
+    int stage, VjpGrid g, double eps, double *__restrict__ exx, double *__restrict__ exy,
     AdjScalars *sc, double *partials) {
     constexpr int NT = TX * TY;
-    constexpr int NW = NT / 64;
     static_assert(NT % 64 == 0 && TX >= 5 && TY >= 5, "tile shape");
     static_assert(5 * (TX + TY) + TX + TY <= NT, "one halo role per thread");
     __shared__ VjpSmem<TX, TY> sm;
@@ -206,13 +263,12 @@ __global__ __launch_bounds__(TX * TY, 2) void k_vjp_tile(
     const int ntx = nx / TX, nty = ny / TY;
     const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
     const int bx = tile % ntx, by = (tile / ntx) % nty, bz = tile / (ntx * nty);
-    const int x0 = bx * TX, y0 = by * TY, z0 = bz * g.kchunk;
-    const int z1 = bz == g.nchunks - 1 ? nz : z0 + g.kchunk;
+    const int x0 = bx * TX, y0 = by * TY, z0 = range_lo(bz, g.nxy, nz), z1 = range_lo(bz + 1, g.nxy, nz);
     const int tid = threadIdx.x, tx = tid % TX, ty = tid / TX;
     const double dt = rec->dt;
     const double alpha = stage == 1 ? rec->a1 : (stage == 2 ? rec->a2 : rec->a3);
     const double coef = (stage == 1 ? 1.0 : (stage == 2 ? 0.25 : 2.0 / 3.0)) * dt;
-    const double rdx = 1.0 / g.dx, rdy = 1.0 / g.dy, rdz = 1.0 / g.dz;
+    const double rdx = 1.0 / g.dx, rdy = 1.0 / g.dy;
     const unsigned plane = static_cast<unsigned>(nx) * static_cast<unsigned>(ny);
     const unsigned col = static_cast<unsigned>(x0 + tx) + static_cast<unsigned>(nx) * (y0 + ty);
 
@@ -247,195 +303,201 @@ __global__ __launch_bounds__(TX * TY, 2) void k_vjp_tile(
             lcol = static_cast<unsigned>(x0 + lc) + static_cast<unsigned>(nx) * wrapi(y0 + TY, ny);
         }
     }
-    auto zoff = [&](int k) { return plane * static_cast<unsigned>(wrapi(k, nz)); };
-
-    // z queues: planes k-2..k+3 (index 0..5) of the own column; az: z-face
-    // contributions to nodes k-2..k+3; dq: pass-through + x/y part of nodes k-2, k-1
-    double fpq[6], fmq[6], uq[6], az[6], dq[2] = {0.0, 0.0};
-#pragma unroll
-    for (int q = 0; q < 5; q++) {
-        uq[q] = u[col + zoff(z0 - 2 + q)];
-        lf_split(uq[q], alpha, fpq[q], fmq[q]);
+    // outflow role (after the faces): 5 x targets per row, then 5 y targets per column
+    int ot = 0;
+    unsigned oidx = 0;
+    if (tid < 5 * TY) {
+        const int r = tid / 5, c5 = tid % 5;
+        ot = c5 < 2 ? c5 - 2 : TX + c5 - 2;
+        oidx = static_cast<unsigned>(wrapi(x0 + ot, nx)) + static_cast<unsigned>(nx) * (y0 + r);
+    } else if (has_halo) {
+        const int h = tid - 5 * TY, cc = h % TX, q = h / TX;
+        ot = q < 2 ? q - 2 : TY + q - 2;
+        oidx = static_cast<unsigned>(x0 + cc) + static_cast<unsigned>(nx) * wrapi(y0 + ot, ny);
     }
-#pragma unroll
-    for (int q = 0; q < 6; q++) az[q] = 0.0;
-    double ub0 = ubo[col + zoff(z0)];
-    double u_next = u[col + zoff(z0 + 3)];
-    double h_next = has_halo ? u[hcol + zoff(z0)] : 0.0;
-    double ub_next = ubo[col + zoff(z0 + 1)];
-    double lh_next = has_lh ? ubo[lcol + zoff(z0)] : 0.0;
-    double ac_next = stage == 3 ? 0.0 : acc[col + zoff(z0)];
-    double sL = 0.0, sLa = 0.0;
 
     // sum over own faces of the contributions to target node tt (tile-relative,
-    // -2..TX+2) of row r, from Gx (AXIS 0) or Gy (AXIS 1), in a fixed order
-    auto gather = [&](const double (&G)[6][TY][TX], int r, int tt, bool xdir) {
+    // -2..T+2) of row / column r, in a fixed order
+    auto gather_x = [&](int r, int tt) {
         double s = 0.0;
 #pragma unroll
         for (int t = 0; t < 6; t++) {
-            const int f = tt + 2 - t;  // face index whose node t is tt
-            const int lim = xdir ? TX : TY;
-            if (f >= 0 && f < lim) s += xdir ? G[t][r][f] : G[t][f][r];
+            const int f = tt + 2 - t;
+            if (f >= 0 && f < TX) s += sm.Gx[t][r][f];
+        }
+        return s;
+    };
+    auto gather_y = [&](int r, int tt) {
+        double s = 0.0;
+#pragma unroll
+        for (int t = 0; t < 6; t++) {
+            const int f = tt + 2 - t;
+            if (f >= 0 && f < TY) s += sm.Gy[t][f][r];
         }
         return s;
     };
 
+    double sL = 0.0, sLa = 0.0;
+    double u_next = u[col + plane * static_cast<unsigned>(z0)];
+    double h_next = has_halo ? u[hcol + plane * static_cast<unsigned>(z0)] : 0.0;
+    double ub_next = ubo[col + plane * static_cast<unsigned>(z0)];
+    double lh_next = has_lh ? ubo[lcol + plane * static_cast<unsigned>(z0)] : 0.0;
+    double ac_next = stage == 3 ? 0.0 : acc[col + plane * static_cast<unsigned>(z0)];
     for (int k = z0; k < z1; k++) {
-        const double un = u_next, hv = h_next, ub1 = ub_next, lh = lh_next, ac = ac_next;
+        const double uv = u_next, hv = h_next, ub = ub_next, lh = lh_next, ac = ac_next;
         if (k + 1 < z1) {
-            u_next = u[col + zoff(k + 4)];
-            if (has_halo) h_next = u[hcol + zoff(k + 1)];
-            ub_next = ubo[col + zoff(k + 2)];
-            if (has_lh) lh_next = ubo[lcol + zoff(k + 1)];
-            if (stage != 3) ac_next = acc[col + zoff(k + 1)];
+            const unsigned pk = plane * static_cast<unsigned>(k + 1);
+            u_next = u[col + pk];
+            if (has_halo) h_next = u[hcol + pk];
+            ub_next = ubo[col + pk];
+            if (has_lh) lh_next = ubo[lcol + pk];
+            if (stage != 3) ac_next = acc[col + pk];
         }
-        uq[5] = un;
-        lf_split(un, alpha, fpq[5], fmq[5]);
-        sm.P[ty + 2][tx + 2] = make_double2(fpq[2], fmq[2]);
-        sm.U[ty + 2][tx + 2] = uq[2];
-        sm.Lm[ty][tx] = coef * ub0;
+        {
+            double fp, fm;
+            lf_split(uv, alpha, fp, fm);
+            sm.Pp[ty + 2][tx + 2] = fp;
+            sm.Pm[ty + 2][tx + 2] = fm;
+            sm.U[ty + 2][tx + 2] = uv;
+            sm.Lm[ty][tx] = coef * ub;
+        }
         if (has_halo) {
             double hp, hm;
             lf_split(hv, alpha, hp, hm);
-            sm.P[hr][hc] = make_double2(hp, hm);
+            sm.Pp[hr][hc] = hp;
+            sm.Pm[hr][hc] = hm;
             sm.U[hr][hc] = hv;
         }
         if (has_lh) sm.Lm[lr][lc] = coef * lh;
         __syncthreads();
 
-        double c[6];
         {  // x face x0+tx+1/2: nodes tile-x tx-2..tx+3 = LDS cols tx..tx+5
-            double fp[6], fm[6], uu[6];
-#pragma unroll
-            for (int t = 0; t < 6; t++) {
-                const double2 pv = sm.P[ty + 2][tx + t];
-                fp[t] = pv.x;
-                fm[t] = pv.y;
-                uu[t] = sm.U[ty + 2][tx + t];
-            }
             const double Fb = (sm.Lm[ty][tx + 1] - sm.Lm[ty][tx]) * rdx;
-            face_vjp<WENO, P>(fp, fm, uu, Fb, alpha, eps, c, sL, sLa);
-#pragma unroll
-            for (int t = 0; t < 6; t++) sm.Gx[t][ty][tx] = c[t];
+            face_vjp_lds<WENO, P>(&sm.Pp[ty + 2][tx], &sm.Pm[ty + 2][tx], &sm.U[ty + 2][tx], 1, Fb, alpha, eps,
+                                  &sm.Gx[0][ty][tx], TX * TY, sL, sLa);
         }
+        __builtin_amdgcn_sched_barrier(0);
         {  // y face y0+ty+1/2: LDS rows ty..ty+5
-            double fp[6], fm[6], uu[6];
-#pragma unroll
-            for (int t = 0; t < 6; t++) {
-                const double2 pv = sm.P[ty + t][tx + 2];
-                fp[t] = pv.x;
-                fm[t] = pv.y;
-                uu[t] = sm.U[ty + t][tx + 2];
-            }
             const double Fb = (sm.Lm[ty + 1][tx] - sm.Lm[ty][tx]) * rdy;
-            face_vjp<WENO, P>(fp, fm, uu, Fb, alpha, eps, c, sL, sLa);
-#pragma unroll
-            for (int t = 0; t < 6; t++) sm.Gy[t][ty][tx] = c[t];
-        }
-        {  // z face k+1/2 from the register queues
-            const double Fb = (coef * ub1 - coef * ub0) * rdz;
-            face_vjp<WENO, P>(fpq, fmq, uq, Fb, alpha, eps, c, sL, sLa);
-#pragma unroll
-            for (int t = 0; t < 6; t++) az[t] += c[t];
+            face_vjp_lds<WENO, P>(&sm.Pp[ty][tx + 2], &sm.Pm[ty][tx + 2], &sm.U[ty][tx + 2], TX + 5, Fb, alpha,
+                                  eps, &sm.Gy[0][ty][tx], TX * TY, sL, sLa);
         }
         __syncthreads();
 
-        // own node (plane k): pass-through and the x/y faces of the tile
-        const unsigned o = col + plane * static_cast<unsigned>(k);
+        const unsigned pk = plane * static_cast<unsigned>(k);
+        const unsigned o = col + pk;
         double pass;
         if (stage == 3) {
-            pass = (2.0 / 3.0) * ub0;
-            acc[o] = ub0 * (1.0 / 3.0);
+            pass = (2.0 / 3.0) * ub;
+            acc[o] = ub * (1.0 / 3.0);
         } else if (stage == 2) {
-            pass = 0.25 * ub0;
-            acc[o] = ac + 0.75 * ub0;
+            pass = 0.25 * ub;
+            acc[o] = ac + 0.75 * ub;
         } else {
-            pass = ac + ub0;
+            pass = ac + ub;
         }
-        const double pxy = pass + (gather(sm.Gx, ty, tx, true) + gather(sm.Gy, tx, ty, false));
-        if (fabs(uq[2]) == alpha) atomicMin(&sc->idx[stage - 1], o);
-        // x/y contributions leaving the tile (by the halo-role threads)
-        if (tid < 5 * TY) {
-            const int r = tid / 5, c5 = tid % 5;
-            const int tt = c5 < 2 ? c5 - 2 : TX + c5 - 2;
-            exx[static_cast<unsigned>(wrapi(x0 + tt, nx)) + static_cast<unsigned>(nx) * (y0 + r) +
-                plane * static_cast<unsigned>(k)] = gather(sm.Gx, r, tt, true);
-        } else if (has_halo) {
-            const int h = tid - 5 * TY, cc = h % TX, q = h / TX;
-            const int tt = q < 2 ? q - 2 : TY + q - 2;
-            exy[static_cast<unsigned>(x0 + cc) + static_cast<unsigned>(nx) * wrapi(y0 + tt, ny) +
-                plane * static_cast<unsigned>(k)] = gather(sm.Gy, cc, tt, false);
-        }
-        // z: node k-2 has all its own faces now
-        if (k - 2 >= z0)
-            ubi[col + plane * static_cast<unsigned>(k - 2)] = dq[0] + az[0];
-        else
-            exz[col + zoff(k - 2)] = az[0];
-        dq[0] = dq[1];
-        dq[1] = pxy;
-#pragma unroll
-        for (int q = 0; q < 5; q++) {
-            az[q] = az[q + 1];
-            fpq[q] = fpq[q + 1];
-            fmq[q] = fmq[q + 1];
-            uq[q] = uq[q + 1];
-        }
-        az[5] = 0.0;
-        ub0 = ub1;
+        ubi[o] = pass + (gather_x(ty, tx) + gather_y(tx, ty));
+        if (fabs(uv) == alpha) atomicMin(&sc->idx[stage - 1], o);
+        if (tid < 5 * TY)
+            exx[oidx + pk] = gather_x(tid / 5, ot);
+        else if (has_halo)
+            exy[oidx + pk] = gather_y((tid - 5 * TY) % TX, ot);
     }
-    // nodes z1-2, z1-1 (their faces z1, z1+1 belong to the chunk above) and
-    // the contributions to z1..z1+2
-    ubi[col + plane * static_cast<unsigned>(z1 - 2)] = dq[0] + az[0];
-    ubi[col + plane * static_cast<unsigned>(z1 - 1)] = dq[1] + az[1];
-#pragma unroll
-    for (int q = 2; q < 5; q++) exz[col + zoff(z1 + q - 2)] = az[q];
-
-    // block partial sums of <lam, L> and <lam, dL/dalpha> (fixed order)
-    const int wave = tid >> 6, lane = tid & 63;
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        sL += __shfl_xor(sL, off, 64);
-        sLa += __shfl_xor(sLa, off, 64);
-    }
-    if (lane == 0) {
-        sm.red[0][wave] = sL;
-        sm.red[1][wave] = sLa;
-    }
-    __syncthreads();
-    if (tid == 0) {
-        double a = 0.0, b = 0.0;
-        for (int w = 0; w < NW; w++) {
-            a += sm.red[0][w];
-            b += sm.red[1][w];
-        }
-        partials[2 * blockIdx.x] = a;
-        partials[2 * blockIdx.x + 1] = b;
-    }
+    block_partials<NT>(sL, sLa, sm.red, partials);
 }
 
-// Adds the exchange fields to the nodes that received contributions from a
-// neighbouring tile or chunk: x offsets 0..2 and TX-2..TX-1 within the tile
-// (exx), the same in y (exy), and planes z0..z0+2, z1-2..z1-1 of the chunk
-// (exz).  Every such node's exchange value was written by exactly one block.
-template <int TX, int TY>
-__global__ __launch_bounds__(256) void k_vjp_inflow(double *ubi, const double *__restrict__ exx,
-                                                    const double *__restrict__ exy, const double *__restrict__ exz,
-                                                    VjpGrid g) {
+// z part of the stage VJP, one thread per column and z chunk (lanes along x):
+// marches the chunk's own z faces z0..z1-1 (+1/2) with register queues of the
+// split fluxes and u of planes k-2..k+3 and adds, for each node of the chunk,
+// the contributions of its own z faces and the x/y exchange values k_vjp_xy
+// left for it (x offsets 0..2 and TX-2..TX-1 within its tile, the same in
+// y).  Node k-2 is finished once its last own face, k+1/2, is done;
+// contributions to nodes of the chunks below (z0-2, z0-1) and above
+// (z1..z1+2) go to exz, which k_vjp_zinflow adds.
+template <int WENO, int P, int TX, int TY>
+__global__ __launch_bounds__(256, 4) void k_vjp_z(const double *__restrict__ u, const double *__restrict__ ubo,
+                                               double *ubi, const StepRec *rec, int stage, VjpGrid g, double eps,
+                                               const double *__restrict__ exx, const double *__restrict__ exy,
+                                               double *__restrict__ exz, double *partials) {
+    __shared__ double red[2][4];
     const unsigned plane = static_cast<unsigned>(g.nx) * static_cast<unsigned>(g.ny);
-    const size_t n = static_cast<size_t>(plane) * g.nz;
-    for (size_t o = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x; o < n; o += 256ull * gridDim.x) {
-        const unsigned x = static_cast<unsigned>(o % g.nx), y = static_cast<unsigned>((o / g.nx) % g.ny);
-        const int z = static_cast<int>(o / plane);
-        const int xi = x % TX, yi = y % TY;
-        const int bz = min(z / g.kchunk, g.nchunks - 1);
-        const int z0 = bz * g.kchunk, z1 = bz == g.nchunks - 1 ? g.nz : z0 + g.kchunk;
-        const bool ix = xi <= 2 || xi >= TX - 2, iy = yi <= 2 || yi >= TY - 2, iz = z - z0 <= 2 || z1 - z <= 2;
-        if (!(ix || iy || iz)) continue;
-        double s = 0.0;
-        if (ix) s += exx[o];
-        if (iy) s += exy[o];
-        if (iz) s += exz[o];
-        ubi[o] += s;
+    const size_t gid = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x;
+    double sL = 0.0, sLa = 0.0;
+    if (gid < static_cast<size_t>(plane) * g.nzc) {
+        const unsigned col = static_cast<unsigned>(gid % plane);
+        const int bz = static_cast<int>(gid / plane);
+        const int nz = g.nz, z0 = range_lo(bz, g.nzc, nz), z1 = range_lo(bz + 1, g.nzc, nz);
+        const int x = static_cast<int>(col % g.nx), y = static_cast<int>(col / g.nx);
+        const bool ix = x % TX <= 2 || x % TX >= TX - 2, iy = y % TY <= 2 || y % TY >= TY - 2;
+        const double dt = rec->dt;
+        const double alpha = stage == 1 ? rec->a1 : (stage == 2 ? rec->a2 : rec->a3);
+        const double coef = (stage == 1 ? 1.0 : (stage == 2 ? 0.25 : 2.0 / 3.0)) * dt;
+        const double rdz = 1.0 / g.dz;
+        auto zo = [&](int k) { return col + plane * static_cast<unsigned>(wrapi(k, nz)); };
+        auto finish = [&](int k, double zc) {
+            const unsigned o = col + plane * static_cast<unsigned>(k);
+            double s = zc;
+            if (ix) s += exx[o];
+            if (iy) s += exy[o];
+            ubi[o] += s;
+        };
+        double fpq[6], fmq[6], uq[6], az[6];
+#pragma unroll
+        for (int q = 0; q < 5; q++) {
+            uq[q] = u[zo(z0 - 2 + q)];
+            lf_split(uq[q], alpha, fpq[q], fmq[q]);
+        }
+#pragma unroll
+        for (int q = 0; q < 6; q++) az[q] = 0.0;
+        double l0 = coef * ubo[zo(z0)];
+        double u_next = u[zo(z0 + 3)], ub_next = ubo[zo(z0 + 1)];
+        for (int k = z0; k < z1; k++) {
+            uq[5] = u_next;
+            const double l1 = coef * ub_next;
+            if (k + 1 < z1) {
+                u_next = u[zo(k + 4)];
+                ub_next = ubo[zo(k + 2)];
+            }
+            lf_split(uq[5], alpha, fpq[5], fmq[5]);
+            double c[6];
+            face_vjp<WENO, P, false>(fpq, fmq, uq, (l1 - l0) * rdz, alpha, eps, c, sL, sLa);
+#pragma unroll
+            for (int t = 0; t < 6; t++) az[t] += c[t];
+            if (k - 2 >= z0)
+                finish(k - 2, az[0]);
+            else
+                exz[zo(k - 2)] = az[0];
+#pragma unroll
+            for (int q = 0; q < 5; q++) {
+                az[q] = az[q + 1];
+                fpq[q] = fpq[q + 1];
+                fmq[q] = fmq[q + 1];
+                uq[q] = uq[q + 1];
+            }
+            az[5] = 0.0;
+            l0 = l1;
+        }
+        finish(z1 - 2, az[0]);
+        finish(z1 - 1, az[1]);
+#pragma unroll
+        for (int q = 2; q < 5; q++) exz[zo(z1 + q - 2)] = az[q];
+    }
+    block_partials<256>(sL, sLa, red, partials);
+}
+
+// Adds exz to the planes that received z contributions from a neighbouring
+// chunk: z0..z0+2 and z1-2..z1-1 of every chunk (5 planes per chunk, each
+// written by exactly one chunk).
+__global__ __launch_bounds__(256) void k_vjp_zinflow(double *ubi, const double *__restrict__ exz, VjpGrid g) {
+    const unsigned plane = static_cast<unsigned>(g.nx) * static_cast<unsigned>(g.ny);
+    const size_t n = static_cast<size_t>(plane) * 5 * g.nzc;
+    for (size_t i = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x; i < n; i += 256ull * gridDim.x) {
+        const unsigned col = static_cast<unsigned>(i % plane);
+        const int p = static_cast<int>(i / plane), bz = p / 5, j = p % 5;
+        const int z0 = range_lo(bz, g.nzc, g.nz), z1 = range_lo(bz + 1, g.nzc, g.nz);
+        const int z = j < 3 ? z0 + j : z1 - 5 + j;
+        const unsigned o = col + plane * static_cast<unsigned>(z);
+        ubi[o] += exz[o];
     }
 }
 
@@ -517,23 +579,72 @@ __global__ void k_sum_J(const double *partials, int n, AdjScalars *sc) {
 namespace {
 
 constexpr int kMisfitBlocks = 1024;
-constexpr int kVjpTX = 16, kVjpTY = 16, kVjpChunk = 16;
+constexpr int kVjpTX = 16, kVjpTY = 16;  // x/y tile
 
-// z chunks of kVjpChunk planes, the last one taking the remainder (nz >= 5).
-VjpGrid vjp_grid(int nx, int ny, int nz, double dx, double dy, double dz) {
+// Number of plane ranges for `units` blocks per range: the smallest count in
+// [1, max_n] whose block total fills the resident capacity R to 95 % or more
+// (whole waves of R), else the best filling one.
+int fill_ranges(size_t units, int max_n, size_t R) {
+    int best = 1;
+    double best_eff = 0.0;
+    for (int n = 1; n <= max_n; n++) {
+        const size_t b = units * n;
+        const double eff = static_cast<double>(b) / (static_cast<double>((b + R - 1) / R) * R);
+        if (eff >= 0.95) return n;
+        if (eff > best_eff + 1e-9) {
+            best_eff = eff;
+            best = n;
+        }
+    }
+    return best;
+}
+
+template <int WENO, int P>
+size_t resident_blocks(const void *kernel, int threads) {
+    int dev = 0, cus = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) !=
+                                                 hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, threads, 0) != hipSuccess || per < 1)
+        return 1024;  // not known: ranges then simply fill 1024 blocks
+    return static_cast<size_t>(cus) * per;
+}
+
+// Plane ranges sized so each launch fills the device's resident blocks.
+template <int WENO, int P>
+VjpGrid vjp_grid_t(int nx, int ny, int nz, double dx, double dy, double dz) {
     VjpGrid g{};
     g.nx = nx;
     g.ny = ny;
     g.nz = nz;
-    g.kchunk = kVjpChunk;
-    g.nchunks = std::max(1, nz / kVjpChunk);
     g.dx = dx;
     g.dy = dy;
     g.dz = dz;
+    const size_t tiles = static_cast<size_t>(nx / kVjpTX) * (ny / kVjpTY);
+    const size_t Rxy = resident_blocks<WENO, P>(reinterpret_cast<const void *>(&k_vjp_xy<WENO, P, kVjpTX, kVjpTY>),
+                                                kVjpTX * kVjpTY);
+    g.nxy = fill_ranges(tiles, nz, Rxy);
+    const size_t cols = static_cast<size_t>(nx) * ny;  // z blocks of 256 columns
+    const size_t Rz = resident_blocks<WENO, P>(reinterpret_cast<const void *>(&k_vjp_z<WENO, P, kVjpTX, kVjpTY>), 256);
+    g.nzc = cols % 256 == 0 ? fill_ranges(cols / 256, std::max(1, nz / 5), Rz) : std::max(1, nz / 8);
     return g;
 }
 
-int vjp_blocks(const VjpGrid &g) { return (g.nx / kVjpTX) * (g.ny / kVjpTY) * g.nchunks; }
+VjpGrid vjp_grid(const mgad_scheme &sc, int nx, int ny, int nz, double dx, double dy, double dz) {
+    if (sc.weno == MGAD_WENO_JS) return vjp_grid_t<kWenoJS, 2>(nx, ny, nz, dx, dy, dz);
+    if (sc.p == 2) return vjp_grid_t<kWenoZ, 2>(nx, ny, nz, dx, dy, dz);
+    return vjp_grid_t<kWenoZ, 1>(nx, ny, nz, dx, dy, dz);
+}
+
+int vjp_xy_blocks(const VjpGrid &g) { return (g.nx / kVjpTX) * (g.ny / kVjpTY) * g.nxy; }
+int vjp_z_blocks(const VjpGrid &g) {
+    return static_cast<int>((static_cast<size_t>(g.nx) * g.ny * g.nzc + 255) / 256);
+}
+// Upper bound of the partial-sum slots over every choice of ranges.
+size_t vjp_max_partials(int nx, int ny, int nz) {
+    const size_t xy = static_cast<size_t>(nx / kVjpTX) * (ny / kVjpTY) * nz;
+    const size_t z = (static_cast<size_t>(nx) * ny * std::max(1, nz / 5) + 255) / 256;
+    return 2 * (xy + z);
+}
 
 template <int WENO, int P>
 int launch_stage_vjp(hipStream_t s, int stage, const double *uin, const double *ubo, double *ubi, double *acc,
@@ -541,12 +652,15 @@ int launch_stage_vjp(hipStream_t s, int stage, const double *uin, const double *
                      bool clipped_last) {
     const size_t n = static_cast<size_t>(g.nx) * g.ny * g.nz;
     double *exx = ex, *exy = ex + n, *exz = ex + 2 * n;
-    const int nb = vjp_blocks(g);
-    hipLaunchKernelGGL((k_vjp_tile<WENO, P, kVjpTX, kVjpTY>), dim3(nb), dim3(kVjpTX * kVjpTY), 0, s, uin, ubo, ubi,
-                       acc, rec, stage, g, eps, exx, exy, exz, sc, partials);
-    const unsigned ib = static_cast<unsigned>(std::min<size_t>((n + 255) / 256, 8192));
-    hipLaunchKernelGGL((k_vjp_inflow<kVjpTX, kVjpTY>), dim3(ib), dim3(256), 0, s, ubi, exx, exy, exz, g);
-    hipLaunchKernelGGL(k_adj_scalars, dim3(1), dim3(256), 0, s, partials, nb, rec, stage, ubi, uin, sc,
+    const int nxy = vjp_xy_blocks(g), nzb = vjp_z_blocks(g);
+    hipLaunchKernelGGL((k_vjp_xy<WENO, P, kVjpTX, kVjpTY>), dim3(nxy), dim3(kVjpTX * kVjpTY), 0, s, uin, ubo, ubi,
+                       acc, rec, stage, g, eps, exx, exy, sc, partials);
+    hipLaunchKernelGGL((k_vjp_z<WENO, P, kVjpTX, kVjpTY>), dim3(nzb), dim3(256), 0, s, uin, ubo, ubi, rec, stage, g,
+                       eps, exx, exy, exz, partials + 2 * nxy);
+    const size_t nin = static_cast<size_t>(g.nx) * g.ny * 5 * g.nzc;
+    hipLaunchKernelGGL(k_vjp_zinflow, dim3(static_cast<unsigned>(std::min<size_t>((nin + 255) / 256, 8192))),
+                       dim3(256), 0, s, ubi, exz, g);
+    hipLaunchKernelGGL(k_adj_scalars, dim3(1), dim3(256), 0, s, partials, nxy + nzb, rec, stage, ubi, uin, sc,
                        clipped_last ? 1 : 0);
     return check_launch();
 }
@@ -586,8 +700,7 @@ AdjLayout layout(unsigned nx, unsigned ny, unsigned nz, int max_steps, int every
     L.off_rec = o; o += align256(L.K * sizeof(StepRec));
     L.off_ctrl = o; o += align256(MGAD_CTRL_BYTES);
     L.off_sc = o; o += align256(sizeof(AdjScalars));
-    const int nb = vjp_blocks(vjp_grid(nx, ny, nz, 1.0, 1.0, 1.0));
-    L.off_part = o; o += align256(sizeof(double) * std::max<size_t>(2 * nb, kMisfitBlocks));
+    L.off_part = o; o += align256(sizeof(double) * std::max<size_t>(vjp_max_partials(nx, ny, nz), kMisfitBlocks));
     L.off_tchk = o; o += align256(sizeof(double) * L.nchk);
     L.total = o;
     return L;
@@ -631,29 +744,24 @@ extern "C" int mgad_burgers_adjoint(hipStream_t stream, const mgad_scheme *schem
     AdjScalars *scal = reinterpret_cast<AdjScalars *>(ws + L.off_sc);
     double *partials = reinterpret_cast<double *>(ws + L.off_part);
     double *ex = reinterpret_cast<double *>(ws + L.off_ex);
-    const VjpGrid vg = vjp_grid(static_cast<int>(nx), static_cast<int>(ny), static_cast<int>(nz), dx, dy, dz);
+    const VjpGrid vg = vjp_grid(*scheme, static_cast<int>(nx), static_cast<int>(ny), static_cast<int>(nz), dx, dy, dz);
     const size_t bytes = L.n * sizeof(double);
     const int K = checkpoint_every;
 
     // Runs steps [0, len) of a segment from S0(0), keeping every stage input
     // (S0(m), S1(m), S2(m)) and the step record; steps past tend are no-ops.
     auto run_segment = [&](int len) {
-        int r = 0;
+        hipLaunchKernelGGL(k_ctrl_rec, dim3(1), dim3(1), 0, stream, static_cast<Ctrl *>(ctrl), rec);
+        int r = check_launch();
         for (int m = 0; m < len && !r; m++) {
             r = burgers_stage_launch_t<double>(stream, *scheme, nx, ny, nz, dx, dy, dz, 1, S0(m), S0(m), S1(m), ctrl,
                                                0, 0);
-            if (!r) {
-                hipLaunchKernelGGL(k_record, dim3(1), dim3(1), 0, stream, static_cast<const Ctrl *>(ctrl), rec + m, 1,
-                                   cfl * dx);
+            if (!r)
                 r = burgers_stage_launch_t<double>(stream, *scheme, nx, ny, nz, dx, dy, dz, 2, S1(m), S0(m), S2(m),
                                                    ctrl, 0, 0);
-            }
-            if (!r) {
-                hipLaunchKernelGGL(k_record, dim3(1), dim3(1), 0, stream, static_cast<const Ctrl *>(ctrl), rec + m, 2,
-                                   cfl * dx);
+            if (!r)
                 r = burgers_stage_launch_t<double>(stream, *scheme, nx, ny, nz, dx, dy, dz, 3, S2(m), S0(m), S0(m + 1),
                                                    ctrl, 0, 0);
-            }
         }
         return r;
     };

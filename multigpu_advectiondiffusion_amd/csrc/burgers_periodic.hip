@@ -80,6 +80,8 @@ __global__ void k_ctrl_init(Ctrl *c, double t0, double tend, double cfl) {
     c->amax[1] = 0ull;
     c->amax[2] = 0ull;
     c->steps = 0ull;
+    c->rec = nullptr;
+    c->rec_step0 = 0ull;
 }
 
 // ---------------------------------------------------------------------------

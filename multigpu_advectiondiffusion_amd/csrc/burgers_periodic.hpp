@@ -8,6 +8,13 @@
 
 namespace mgad {
 
+// Per-step record of a run (kept by the adjoint): dt, the LF alpha of each
+// stage and the unclipped CFL step cfl*dx/a1 (a clipped last step has
+// dt != dt_cfl).
+struct StepRec {
+    double dt, a1, a2, a3, dt_cfl;
+};
+
 // Layout of the caller's MGAD_CTRL_BYTES device buffer.
 struct Ctrl {
     double t;                    // time at the start of the next step
@@ -16,6 +23,8 @@ struct Ctrl {
     double cfl;
     unsigned long long amax[3];  // max|u| of: [0] u^n, [1] u^(1), [2] u^(2)
     unsigned long long steps;
+    StepRec *rec;                // if set, step s records into rec[s - rec_step0]
+    unsigned long long rec_step0;
 };
 static_assert(sizeof(Ctrl) <= MGAD_CTRL_BYTES, "ctrl layout");
 
@@ -38,7 +47,8 @@ __device__ __forceinline__ unsigned xcd_tile(unsigned bid, unsigned nblocks) {
 // Step control shared by every block of a stage.  Stage 1 derives dt from
 // max|u^n| (all blocks compute the same value) and records it; stage 2 moves
 // t forward.  Each stage's lead thread clears the max|u| slot the next-but-one
-// stage accumulates into.  Returns false once the run has reached tend.
+// stage accumulates into, and fills the step record when one is kept.
+// Returns false once the run has reached tend.
 __device__ __forceinline__ bool stage_control(Ctrl *ctrl, int stage, double dx, bool lead, double &dt,
                                               double &alpha) {
     if (stage == 1) {
@@ -54,6 +64,12 @@ __device__ __forceinline__ bool stage_control(Ctrl *ctrl, int stage, double dx, 
         if (lead) {
             ctrl->dt = dt;
             ctrl->amax[2] = 0ull;
+            if (ctrl->rec) {
+                StepRec &r = ctrl->rec[ctrl->steps - ctrl->rec_step0];
+                r.dt = dt;
+                r.a1 = a;
+                r.dt_cfl = ctrl->cfl * dx / a;
+            }
         }
         return true;
     }
@@ -62,10 +78,12 @@ __device__ __forceinline__ bool stage_control(Ctrl *ctrl, int stage, double dx, 
     alpha = bits_double(ctrl->amax[stage - 1]);
     if (lead) {
         if (stage == 2) {
+            if (ctrl->rec) ctrl->rec[ctrl->steps - ctrl->rec_step0].a2 = alpha;
             ctrl->amax[0] = 0ull;
             ctrl->t = ctrl->t + dt;
             ctrl->steps = ctrl->steps + 1ull;
         } else {
+            if (ctrl->rec) ctrl->rec[ctrl->steps - 1ull - ctrl->rec_step0].a3 = alpha;
             ctrl->amax[1] = 0ull;
         }
     }
