@@ -169,8 +169,9 @@ def main() -> int:
         from multigpu_advectiondiffusion_amd import studies
 
         b = studies.batched(steps=200)
-        out["batched"] = {k: b[k] for k in ("speedup", "batched_gpts", "one_by_one_gpts", "problems", "points",
-                                             "steps", "results_identical")}
+        out["batched"] = {k: b[k] for k in ("speedup", "speedup_vs_eager", "batched_gpts", "one_by_one_gpts",
+                                             "one_by_one_eager_gpts", "problems", "points", "steps",
+                                             "results_identical")}
         print("batched study done", file=sys.stderr, flush=True)
         a = studies.adjoint(n=128)
         out["adjoint"] = {"cost_ratio": a["cost_ratio"], "steps": a["cost_steps"], "grid": 128,

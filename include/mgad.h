@@ -98,6 +98,43 @@ void Call_RK3d(dim3 numBlocks, dim3 threadsPerBlock, hipStream_t aStream,
                unsigned int nz, double dt, double *u, double *uo, double *Lu);
 
 /* ------------------------------------------------------------------------ */
+/* Pitched drop-ins                                                          */
+/* ------------------------------------------------------------------------ */
+/* The reference's pitched-memory variants take the same names with an extra
+ * `pitch` (row length in elements, >= Nx; cudaMallocPitch bytes / sizeof
+ * REAL): node (i, j, k) is at i + pitch*j + pitch*Ny*k.  One library cannot
+ * export two signatures under one name, so they carry a _pitched suffix.
+ * Same update sets and arithmetic as the unpitched entry points above;
+ * padding columns i >= Nx are neither read nor written.
+ *   Burgers: SingleGPU/Burgers3d_WENO5_PitchedMem/Burgers.h:73-77 and
+ *   SingleGPU/Burgers3d_WENO5_SharedMem/Burgers.h:79-83.
+ *   Heat: SingleGPU/Diffusion3d_PitchedMem/acoustics3d.h:58-59 and
+ *   SingleGPU/Diffusion3d_Blocking/acoustics3d.h:59-60. */
+void Call_Adv_x_pitched(dim3 numBlocks, dim3 threadsPerBlock, hipStream_t aStream,
+                        unsigned int pitch, unsigned int Nx, unsigned int Ny,
+                        unsigned int Nz, double dx, double *u, double *Lu);
+void Call_Adv_y_pitched(dim3 numBlocks, dim3 threadsPerBlock, hipStream_t aStream,
+                        unsigned int pitch, unsigned int Nx, unsigned int Ny,
+                        unsigned int Nz, double dy, double *v, double *Lv);
+void Call_Adv_z_pitched(dim3 numBlocks, dim3 threadsPerBlock, hipStream_t aStream,
+                        unsigned int pitch, unsigned int Nx, unsigned int Ny,
+                        unsigned int Nz, double dz, double *w, double *Lw);
+void Call_Diff_pitched(dim3 numBlocks, dim3 threadsPerBlock, hipStream_t aStream,
+                       unsigned int pitch, unsigned int Nx, unsigned int Ny,
+                       unsigned int Nz, double *q, double *Lq);
+void Call_sspRK_pitched(dim3 numBlocks, dim3 threadsPerBlock, hipStream_t aStream,
+                        unsigned int pitch, unsigned int Nx, unsigned int Ny,
+                        unsigned int Nz, unsigned int step, double dt, double *u,
+                        double *uo, double *Lu);
+void Call_Lu3d_pitched(dim3 numBlocks, dim3 threadsPerBlock, hipStream_t aStream,
+                       unsigned int pitch, unsigned int nx, unsigned int ny,
+                       unsigned int nz, double *u, double *Lu);
+void Call_RK3d_pitched(dim3 numBlocks, dim3 threadsPerBlock, hipStream_t aStream,
+                       unsigned int step, unsigned int pitch, unsigned int nx,
+                       unsigned int ny, unsigned int nz, double dt, double *u,
+                       double *uo, double *Lu);
+
+/* ------------------------------------------------------------------------ */
 /* Scheme selection for the Call_Adv_* sweeps                                */
 /* ------------------------------------------------------------------------ */
 #define MGAD_WENO_Z     0  /* SingleGPU/Burgers3d_WENO5/kernels.cu:142-192 */

@@ -22,6 +22,8 @@ EXPORTS = (
     "mgad_get_last_error", "mgad_error_string", "mgad_version",
     "CopyToConstantMemory", "Call_Adv_x", "Call_Adv_y", "Call_Adv_z", "Call_Diff_",
     "Call_sspRK", "Call_Lu3d", "Call_RK3d",
+    "Call_Adv_x_pitched", "Call_Adv_y_pitched", "Call_Adv_z_pitched", "Call_Diff_pitched", "Call_sspRK_pitched",
+    "Call_Lu3d_pitched", "Call_RK3d_pitched",
     "mgad_set_adv_scheme", "mgad_lu3d_o2",
     "mgad_burgers_init", "mgad_burgers_stage", "mgad_burgers_steps", "mgad_burgers_read_ctrl",
     "mgad_burgers_init_p", "mgad_burgers_steps_p", "mgad_burgers_autotune", "mgad_burgers_set_launch",
@@ -94,6 +96,18 @@ def lib():
         L.Call_Lu3d.restype = None
         L.Call_RK3d.argtypes = [Dim3, Dim3, VP, U, U, U, U, D, VP, VP, VP]
         L.Call_RK3d.restype = None
+        for name in ("Call_Adv_x_pitched", "Call_Adv_y_pitched", "Call_Adv_z_pitched"):
+            f = getattr(L, name)
+            f.argtypes = [Dim3, Dim3, VP, U, U, U, U, D, VP, VP]
+            f.restype = None
+        L.Call_Diff_pitched.argtypes = [Dim3, Dim3, VP, U, U, U, U, VP, VP]
+        L.Call_Diff_pitched.restype = None
+        L.Call_sspRK_pitched.argtypes = [Dim3, Dim3, VP, U, U, U, U, U, D, VP, VP, VP]
+        L.Call_sspRK_pitched.restype = None
+        L.Call_Lu3d_pitched.argtypes = [Dim3, Dim3, VP, U, U, U, U, VP, VP]
+        L.Call_Lu3d_pitched.restype = None
+        L.Call_RK3d_pitched.argtypes = [Dim3, Dim3, VP, U, U, U, U, U, D, VP, VP, VP]
+        L.Call_RK3d_pitched.restype = None
         L.mgad_set_adv_scheme.argtypes = [I, I, D, I, D]
         L.mgad_set_adv_scheme.restype = I
         L.mgad_lu3d_o2.argtypes = [VP, U, U, U, VP, VP]

@@ -85,6 +85,50 @@ def Call_RK3d(numBlocks, threadsPerBlock, aStream, step, nx, ny, nz, dt, u, uo, 
     _capi.raise_last("Call_RK3d")
 
 
+# Pitched-memory variants (SingleGPU/Burgers3d_WENO5_PitchedMem/Burgers.h:73-77,
+# SingleGPU/Diffusion3d_PitchedMem/acoustics3d.h:58-59): the reference's names
+# and argument order, pitch = row length in elements.
+
+def _call(name, *args):
+    getattr(_capi.lib(), name)(*args)
+    _capi.raise_last(name)
+
+
+def Call_Adv_x_pitched(numBlocks, threadsPerBlock, aStream, pitch, Nx, Ny, Nz, dx, u, Lu) -> None:
+    _call("Call_Adv_x_pitched", _dim(numBlocks), _dim(threadsPerBlock), _stream(aStream), pitch, Nx, Ny, Nz, dx,
+          _ptr(u), _ptr(Lu))
+
+
+def Call_Adv_y_pitched(numBlocks, threadsPerBlock, aStream, pitch, Nx, Ny, Nz, dy, v, Lv) -> None:
+    _call("Call_Adv_y_pitched", _dim(numBlocks), _dim(threadsPerBlock), _stream(aStream), pitch, Nx, Ny, Nz, dy,
+          _ptr(v), _ptr(Lv))
+
+
+def Call_Adv_z_pitched(numBlocks, threadsPerBlock, aStream, pitch, Nx, Ny, Nz, dz, w, Lw) -> None:
+    _call("Call_Adv_z_pitched", _dim(numBlocks), _dim(threadsPerBlock), _stream(aStream), pitch, Nx, Ny, Nz, dz,
+          _ptr(w), _ptr(Lw))
+
+
+def Call_Diff_pitched(numBlocks, threadsPerBlock, aStream, pitch, Nx, Ny, Nz, q, Lq) -> None:
+    _call("Call_Diff_pitched", _dim(numBlocks), _dim(threadsPerBlock), _stream(aStream), pitch, Nx, Ny, Nz, _ptr(q),
+          _ptr(Lq))
+
+
+def Call_sspRK_pitched(numBlocks, threadsPerBlock, aStream, pitch, Nx, Ny, Nz, step, dt, u, uo, Lu) -> None:
+    _call("Call_sspRK_pitched", _dim(numBlocks), _dim(threadsPerBlock), _stream(aStream), pitch, Nx, Ny, Nz, step,
+          dt, _ptr(u), _ptr(uo), _ptr(Lu))
+
+
+def Call_Lu3d_pitched(numBlocks, threadsPerBlock, aStream, pitch, nx, ny, nz, u, Lu) -> None:
+    _call("Call_Lu3d_pitched", _dim(numBlocks), _dim(threadsPerBlock), _stream(aStream), pitch, nx, ny, nz, _ptr(u),
+          _ptr(Lu))
+
+
+def Call_RK3d_pitched(numBlocks, threadsPerBlock, aStream, step, pitch, nx, ny, nz, dt, u, uo, Lu) -> None:
+    _call("Call_RK3d_pitched", _dim(numBlocks), _dim(threadsPerBlock), _stream(aStream), step, pitch, nx, ny, nz,
+          dt, _ptr(u), _ptr(uo), _ptr(Lu))
+
+
 def set_adv_scheme(weno: int = _capi.WENO_Z, p: int = 1, eps: float = 1e-6, lf: int = _capi.LF_LOCAL,
                    alpha: float = 0.0) -> None:
     """Extra Call_Adv_* parameters (defaults = the reference's compile-time choice)."""

@@ -8,9 +8,11 @@ batched: 64 independent periodic WENO5-Z (p=2, eps=1e-40) FP64 problems with
     edge lengths drawn seeded-uniformly from {32,48,64,96,128} (SplitMix64 seed
     4), each with its own Fourier initial field (seed 4+i) and its own dt from
     CFL 0.4, advanced `steps` steps (a) in one persistent launch per RK stage
-    under a hipGraph-captured step and (b) problem by problem with the
-    single-problem kernel (each size autotuned, also under a hipGraph).
-    Reports both throughputs (points * steps / s) and their ratio.
+    under a hipGraph-captured step, (b) problem by problem with the
+    single-problem kernel, each size autotuned and each step a replayed
+    hipGraph, and (c) problem by problem with eager launches of the default
+    launch shape.  Reports the throughputs (points * steps / s) and the
+    ratios of (a) to (b) and to (c).
 fp32: the same Burgers setup with WENO5-Z at n^3, run to t = 0.5 once in FP64
     and once with FP32 storage (FP64 split fluxes, smoothness indicators and
     weights); reports the L1 (mean |diff|) and L-infinity error of the FP32
@@ -62,13 +64,6 @@ def batched(steps: int = 200, nprob: int = 64, scheme: str = "Z2") -> dict:
     sizes = batch_sizes(nprob)
     u0s = [solvers.ic_fourier(n, seed=4 + i) for i, n in enumerate(sizes)]
     b = solvers.BurgersBatch(sizes, scheme=scheme)
-    # autotune the single-problem launch of each distinct size (one-by-one baseline)
-    tuned = {}
-    for n in sorted(set(sizes)):
-        s = solvers.PeriodicBurgers(n, scheme=scheme, stream=b.stream)
-        s.set_initial(u0s[sizes.index(n)])
-        tuned[n] = solvers.autotune(s)
-        del s
     e0, e1 = hip.Event(), hip.Event()
 
     def timed(fn):
@@ -82,11 +77,23 @@ def batched(steps: int = 200, nprob: int = 64, scheme: str = "Z2") -> dict:
         e1.synchronize()
         return e0.elapsed_ms(e1) / 1e3
 
-    t_batch = timed(lambda k: b.steps(k, graph=True))
-    sol_batch = b.solutions()
+    # one by one as a plain caller would run them: default launch shape, eager launches
+    for n in sorted(set(sizes)):
+        solvers.set_launch(scheme, n, 0, 0)
+    t_eager = timed(lambda k: b.steps_serial(k, graph=False))
+    sol_eager = b.solutions()
+    # one by one at its best: each size autotuned, each step a replayed hipGraph
+    tuned = {}
+    for n in sorted(set(sizes)):
+        s = solvers.PeriodicBurgers(n, scheme=scheme, stream=b.stream)
+        s.set_initial(u0s[sizes.index(n)])
+        tuned[n] = solvers.autotune(s)
+        del s
     t_serial = timed(lambda k: b.steps_serial(k, graph=True))
     sol_serial = b.solutions()
-    identical = all(np.array_equal(x, y) for x, y in zip(sol_batch, sol_serial))
+    t_batch = timed(lambda k: b.steps(k, graph=True))
+    sol_batch = b.solutions()
+    identical = all(np.array_equal(x, y) and np.array_equal(x, z) for x, y, z in zip(sol_batch, sol_serial, sol_eager))
     pts = b.npts
     return {
         "study": "batched",
@@ -96,9 +103,12 @@ def batched(steps: int = 200, nprob: int = 64, scheme: str = "Z2") -> dict:
         "steps": steps,
         "batched_gpts": pts * steps / t_batch / 1e9,
         "one_by_one_gpts": pts * steps / t_serial / 1e9,
+        "one_by_one_eager_gpts": pts * steps / t_eager / 1e9,
         "speedup": t_serial / t_batch,
+        "speedup_vs_eager": t_eager / t_batch,
         "batched_s": t_batch,
         "one_by_one_s": t_serial,
+        "one_by_one_eager_s": t_eager,
         "kchunk_batched": b.batch.kchunk,
         "work_items": b.batch.nitems,
         "tuned_single": {str(k): list(v) for k, v in tuned.items()},

@@ -123,3 +123,66 @@ def test_invalid_args_recorded(hipdev):
     with pytest.raises(_capi.MgadError):
         R.Call_Adv_x(None, None, None, 6, 10, 10, 0.1, 0, 0)
     assert _capi.last_error() == 0
+
+
+def padded(a, pitch, seed):
+    """a (nz, ny, nx) stored with row pitch `pitch`; padding columns get random values."""
+    nz, ny, nx = a.shape
+    out = np.random.default_rng(seed).uniform(-5, 5, size=(nz, ny, pitch))
+    out[:, :, :nx] = a
+    return out
+
+
+@pytest.mark.parametrize("shape,pitch", [((24, 20, 18), 32), ((13, 9, 8), 13), ((40, 11, 9), 64)])
+def test_pitched_burgers_dropins(hipdev, shape, pitch):
+    """Pitched variants on a padded array equal the unpitched oracle on the
+    compact one; padding columns are neither read nor written."""
+    nx, ny, nz = shape
+    kx, ky, kz = 0.3, 0.2, 0.1
+    R.CopyToConstantMemory(kx, ky, kz)
+    u, Lu, uo = field(shape, 11), field(shape, 12), field(shape, 13)
+    pu, pLu, puo = padded(u, pitch, 1), padded(Lu, pitch, 2), padded(uo, pitch, 3)
+    du, dLu, duo = DeviceArray.from_host(pu), DeviceArray.from_host(pLu), DeviceArray.from_host(puo)
+    ref = Lu.copy()
+    for axis, (name, d) in enumerate((("Call_Adv_x_pitched", 0.1), ("Call_Adv_y_pitched", 0.07),
+                                      ("Call_Adv_z_pitched", 0.13))):
+        getattr(R, name)(None, None, None, pitch, nx, ny, nz, d, du, dLu)
+        O.adv(axis, u, ref, d, **O.REFERENCE)
+        got = dLu.download()
+        assert np.array_equal(got[:, :, :nx], ref) and np.array_equal(got[:, :, nx:], pLu[:, :, nx:])
+    R.Call_Diff_pitched(None, None, None, pitch, nx, ny, nz, du, dLu)
+    O.diff13_b5(u, ref, kx, ky, kz)
+    got = dLu.download()
+    assert np.array_equal(got[:, :, :nx], ref) and np.array_equal(got[:, :, nx:], pLu[:, :, nx:])
+    for step in (1, 2, 3):
+        R.Call_sspRK_pitched(None, None, None, pitch, nx, ny, nz, step, 0.01, du, duo, dLu)
+        O.ssprk_b5(u, uo, ref, step, 0.01)
+        got = du.download()
+        assert np.array_equal(got[:, :, :nx], u) and np.array_equal(got[:, :, nx:], pu[:, :, nx:])
+
+
+@pytest.mark.parametrize("shape,pitch", [((24, 20, 18), 32), ((5, 6, 7), 8)])
+def test_pitched_heat_dropins(hipdev, shape, pitch):
+    nx, ny, nz = shape
+    kx, ky, kz = 0.05, 0.04, 0.03
+    R.CopyToConstantMemory(kx, ky, kz)
+    u, Lu, uo = field(shape, 14), field(shape, 15), field(shape, 16)
+    pu, pLu, puo = padded(u, pitch, 4), padded(Lu, pitch, 5), padded(uo, pitch, 6)
+    du, dLu, duo = DeviceArray.from_host(pu), DeviceArray.from_host(pLu), DeviceArray.from_host(puo)
+    R.Call_Lu3d_pitched(None, None, None, pitch, nx, ny, nz, du, dLu)
+    ref = Lu.copy()
+    O.lu3d_13(u, ref, kx, ky, kz)
+    got = dLu.download()
+    assert np.array_equal(got[:, :, :nx], ref) and np.array_equal(got[:, :, nx:], pLu[:, :, nx:])
+    for step in (1, 2, 3):
+        R.Call_RK3d_pitched(None, None, None, step, pitch, nx, ny, nz, 0.02, du, duo, dLu)
+        O.rk3d(u, uo, ref, step, 0.02)
+        got = du.download()
+        assert np.array_equal(got[:, :, :nx], u) and np.array_equal(got[:, :, nx:], pu[:, :, nx:])
+
+
+def test_pitched_rejects_short_pitch(hipdev):
+    # buffers hold 16 x 10 x 10, so even an unchecked launch would stay in bounds
+    du, dLu = DeviceArray((10, 10, 16)), DeviceArray((10, 10, 16))
+    with pytest.raises(_capi.MgadError):
+        R.Call_Adv_x_pitched(None, None, None, 15, 16, 10, 10, 0.1, du, dLu)
