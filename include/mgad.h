@@ -373,6 +373,31 @@ int mgad_ic_gauss_noise(double *u, unsigned int nx, unsigned int ny,
                         unsigned int nz, double sigma, double noise,
                         uint64_t seed);
 
+/* SaveBinary3D, SingleGPU/Burgers3d_WENO5/tools.cpp:59-87: writes the host
+ * field as 32-bit floats, x fastest, no header; path NULL writes result.bin
+ * as the reference does. */
+int mgad_save_binary3d(const double *u, unsigned int nx, unsigned int ny,
+                       unsigned int nz, const char *path);
+/* Reads a SaveBinary3D file of exactly nx*ny*nz floats into doubles. */
+int mgad_load_binary3d(double *u, unsigned int nx, unsigned int ny,
+                       unsigned int nz, const char *path);
+/* Call_Init3d, SingleGPU/Burgers3d_WENO5/tools.cpp:92-140 (ic 1 square
+ * jump, 2 zero, 3 Gaussian exp(-|x-1|^2/0.1) at x = i*dx).  Host array. */
+int mgad_init3d_burgers(int ic, double *u0, double dx, double dy, double dz,
+                        unsigned int nx, unsigned int ny, unsigned int nz);
+/* Call_Init3D, SingleGPU/Diffusion3d_baselineCode/tools.cpp:92-143 (ic 1
+ * top/bottom planes at 1, 2 sine mode, 3 heat kernel at t0 = 0.1 centred at
+ * (5,5,5); ic 2, 3 with zero boundary nodes).  Host array. */
+int mgad_init3d_heat(int ic, double *u0, double dx, double dy, double dz,
+                     unsigned int nx, unsigned int ny, unsigned int nz);
+/* CalcError, SingleGPU/Diffusion3d_baselineCode/tools.cpp:156-180, with the
+ * node index corrected: norms of u minus the heat kernel (0.1/t)^(3/2)
+ * exp(-r^2/(4t)) centred at (5,5,5); out = {L1, L2, Linf} with L1, L2
+ * weighted by dx*dy*dz as the reference prints them. */
+int mgad_heat_error_norms(const double *u, double t, double dx, double dy,
+                          double dz, unsigned int nx, unsigned int ny,
+                          unsigned int nz, double *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -30,6 +30,7 @@ EXPORTS = (
     "mgad_batch_workspace_bytes", "mgad_batch_init", "mgad_batch_steps", "mgad_batch_steps_serial",
     "mgad_adjoint_workspace_bytes", "mgad_burgers_adjoint",
     "mgad_heat7_fe_steps", "mgad_check_division", "mgad_ic_fourier", "mgad_ic_gauss_noise",
+    "mgad_save_binary3d", "mgad_load_binary3d", "mgad_init3d_burgers", "mgad_init3d_heat", "mgad_heat_error_norms",
 )
 
 
@@ -147,6 +148,16 @@ def lib():
         L.mgad_heat7_fe_steps.restype = I
         L.mgad_check_division.argtypes = [VP, I, VP, VP, VP, VP, ctypes.c_size_t]
         L.mgad_check_division.restype = I
+        L.mgad_save_binary3d.argtypes = [VP, U, U, U, ctypes.c_char_p]
+        L.mgad_save_binary3d.restype = I
+        L.mgad_load_binary3d.argtypes = [VP, U, U, U, ctypes.c_char_p]
+        L.mgad_load_binary3d.restype = I
+        for name in ("mgad_init3d_burgers", "mgad_init3d_heat"):
+            f = getattr(L, name)
+            f.argtypes = [I, VP, D, D, D, U, U, U]
+            f.restype = I
+        L.mgad_heat_error_norms.argtypes = [VP, D, D, D, D, U, U, U, VP]
+        L.mgad_heat_error_norms.restype = I
         L.mgad_ic_fourier.argtypes = [VP, U, U, U, D, D, ctypes.c_uint64, I, I, D, D]
         L.mgad_ic_fourier.restype = I
         L.mgad_ic_gauss_noise.argtypes = [VP, U, U, U, D, D, ctypes.c_uint64]
