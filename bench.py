@@ -12,7 +12,9 @@ After the timed region (and the CPU baseline) it also runs the three other
 measurements the metric names, reported as extra fields: the batched
 uneven-size speedup over one-by-one launches (64 problems, 200 steps), the
 adjoint cost ratio and finite-difference agreement (128^3, 400 steps / 16
-directions), and the FP32-storage error against FP64 (512^3 to t = 0.5);
+directions), the FP32-storage error against FP64 (512^3 to t = 0.5), and the
+reference's own viscous Burgers runs with the fused reference step against
+the kernel times the reference published for them;
 --quick skips them.  Progress goes to stderr.
 """
 from __future__ import annotations
@@ -182,6 +184,11 @@ def main() -> int:
                           "fd_median_rel_err_h1e-6": max(a["T0.2"]["h1e-06"]["median_rel_err"],
                                                          a["T0.5"]["h1e-06"]["median_rel_err"])}
         print("adjoint study done", file=sys.stderr, flush=True)
+        r = studies.reference_cases()
+        out["reference_cases"] = [{k: c[k] for k in ("grid", "steps", "fused_gpts", "published_gpts",
+                                                     "speedup_vs_published", "dropin_gpts", "fused_vs_dropin")}
+                                  for c in r["cases"]]
+        print("reference cases done", file=sys.stderr, flush=True)
         f = studies.fp32(n=512)
         out["fp32_storage_512"] = {k: f[k] for k in ("l1", "linf", "steps_f64", "gpts_f64", "gpts_f32")}
         print("fp32 study done", file=sys.stderr, flush=True)

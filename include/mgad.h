@@ -329,6 +329,27 @@ int mgad_burgers_read_ctrl(hipStream_t stream, const void *ctrl,
                            mgad_burgers_state *out);
 
 /* ------------------------------------------------------------------------ */
+/* The reference's own Burgers step, fused                                   */
+/* ------------------------------------------------------------------------ */
+
+/* nsteps SSP-RK3 steps of the reference driver's loop, one launch per stage:
+ * the same u as, per step, copying u to uo and running for rk = 1..3
+ *   Call_Adv_x, Call_Adv_y, Call_Adv_z, Call_Diff_ (if viscous),
+ *   Call_sspRK(rk, dt, u, uo, Lu)
+ * (SingleGPU/Burgers3d_WENO5/main.cpp:139-149) with
+ * mgad_set_adv_scheme(scheme->weno, scheme->p, scheme->eps, MGAD_LF_LOCAL)
+ * and the CopyToConstantMemory coefficients, bit for bit.  Interior nodes
+ * 3..n-4 of every axis are updated, the three ghost layers are left as they
+ * are; dt is the caller's fixed step.  w1 and w2 are scratch fields of the
+ * same size (each call first copies u into both, for the ghost layers).
+ * Requires n >= 7 per axis.  No Lu array is produced. */
+int mgad_burgers_ref_steps(hipStream_t stream, const mgad_scheme *scheme,
+                           int viscous, unsigned int nx, unsigned int ny,
+                           unsigned int nz, double dx, double dy, double dz,
+                           double dt, double *u, double *w1, double *w2,
+                           int nsteps, int use_graph);
+
+/* ------------------------------------------------------------------------ */
 /* Heat: 7-point forward Euler                                               */
 /* ------------------------------------------------------------------------ */
 

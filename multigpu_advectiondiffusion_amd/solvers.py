@@ -235,6 +235,36 @@ class BurgersAdjoint:
         return J.value, self.grad.download(), n.value
 
 
+class ReferenceBurgers:
+    """The reference driver's Burgers step (SingleGPU/Burgers3d_WENO5/main.cpp:
+    139-149: local-LF WENO5 sweeps, optional 13-point viscous term with the
+    CopyToConstantMemory coefficients, SSP-RK3 on the interior, frozen 3-cell
+    ghosts) as one fused launch per stage."""
+
+    def __init__(self, shape, dx, dy, dz, scheme: str = "Z1", viscous: bool = True, stream: Stream | None = None):
+        self.nx, self.ny, self.nz = (shape, shape, shape) if np.isscalar(shape) else shape
+        self.dx, self.dy, self.dz = dx, dy, dz
+        self.scheme = scheme_struct(scheme)
+        self.viscous = viscous
+        self.stream = stream if stream is not None else Stream()
+        shp = (self.nz, self.ny, self.nx)
+        self.u, self.w1, self.w2 = DeviceArray(shp), DeviceArray(shp), DeviceArray(shp)
+
+    def set_initial(self, u0: np.ndarray) -> None:
+        self.u.upload(np.ascontiguousarray(u0, dtype=np.float64))
+
+    def steps(self, nsteps: int, dt: float, graph: bool = True) -> None:
+        rc = _capi.lib().mgad_burgers_ref_steps(self.stream.handle, ctypes.byref(self.scheme),
+                                                1 if self.viscous else 0, self.nx, self.ny, self.nz, self.dx,
+                                                self.dy, self.dz, dt, self.u.ptr, self.w1.ptr, self.w2.ptr, nsteps,
+                                                1 if graph else 0)
+        _capi.check(rc, "mgad_burgers_ref_steps")
+
+    def solution(self) -> np.ndarray:
+        self.stream.synchronize()
+        return self.u.download()
+
+
 def heat7_params(shape, C: float = 1.0, L: float = 1.0, r: float = 0.15) -> dict:
     """Node-inclusive spacing dx = L/(n-1) (SingleGPU/Diffusion3d_baselineCode/
     main.cpp:55-57), coefficients kx = C/(12 dx^2) (:59-61), dt = r dx^2 / C."""

@@ -3,6 +3,7 @@
     python -m multigpu_advectiondiffusion_amd.studies batched   [--steps 200]
     python -m multigpu_advectiondiffusion_amd.studies fp32      [--n 512]
     python -m multigpu_advectiondiffusion_amd.studies adjoint   [--n 128]
+    python -m multigpu_advectiondiffusion_amd.studies reference
 
 batched: 64 independent periodic WENO5-Z (p=2, eps=1e-40) FP64 problems with
     edge lengths drawn seeded-uniformly from {32,48,64,96,128} (SplitMix64 seed
@@ -25,6 +26,10 @@ adjoint: J = 1/2 |u(T) - u_target|^2 for WENO5-Z (p=2) Burgers at n^3 FP64,
     along 16 seeded unit directions against central finite differences
     (h = 1e-6, formed point-wise) at T = 0.2 (before shocks) and T = 0.5
     (after).
+reference: the reference's own viscous Burgers runs (512^3 and 1000x1000x200
+    to tEnd 0.1, 1601x986x35 to 0.4) with the fused reference-step kernel, against
+    the kernel seconds the reference published for them, plus the drop-in
+    sequence on the same GPU for a few steps.
 Each prints one JSON line.
 """
 from __future__ import annotations
@@ -212,14 +217,87 @@ def adjoint(n: int = 128, steps: int = 400, ndir: int = 16, h: float = 1e-6) -> 
     return out
 
 
+# The reference's own Burgers runs (SingleGPU/Burgers3d_WENO5/Run.m:1-51):
+# viscous (C = 1e-5) local-LF WENO5-Z p=1 eps=1e-6, frozen 3-cell ghosts,
+# Gaussian initial condition (Call_Init3d ic 3), CFL 0.3, L = W = H = 2,
+# dt = min(CFL dx, 0.9/(2C sum 1/dx^2)), the last step clipped to tEnd
+# (main.cpp:128-136); per run the published kernel seconds and RK3 step
+# count.  Run.m keeps tEnd = 0.1, which gives the 86 and 167 steps printed for
+# the first two grids; the 1067 steps printed for 1601x986x35 are tEnd = 0.4.
+REFERENCE_CASES = [((512, 512, 512), 39.37, 86, 0.1), ((1000, 1000, 200), 247.54, 167, 0.1),
+                   ((1601, 986, 35), 563.49, 1067, 0.4)]
+
+
+def reference_cases(cases=REFERENCE_CASES, dropin_steps: int = 3) -> dict:
+    from . import fileio
+    from . import reference_api as R
+    from .hip import DeviceArray
+
+    out = {"study": "reference_cases", "scheme": "WENO5-Z p=1 eps=1e-6, local LF, 13-point viscous C=1e-5",
+           "cases": []}
+    e0, e1 = hip.Event(), hip.Event()
+    for (nx, ny, nz), published_s, published_steps, tend in cases:
+        L, C, cfl = 2.0, 1e-5, 0.3
+        dx, dy, dz = L / (nx - 1), L / (ny - 1), L / (nz - 1)
+        kc = (C / (12 * dx * dx), C / (12 * dy * dy), C / (12 * dz * dz))
+        R.CopyToConstantMemory(*kc)
+        dt = min(cfl * dx / 1.0, 1.0 / (2 * C * (1 / dx / dx + 1 / dy / dy + 1 / dz / dz)) * 0.9)
+        t, dts = 0.0, []
+        while t < tend:
+            d = tend - t if t + dt > tend else dt
+            dts.append(d)
+            t += d
+        u0 = fileio.init3d_burgers(3, (nx, ny, nz), dx, dy, dz)
+        s = solvers.ReferenceBurgers((nx, ny, nz), dx, dy, dz, scheme="Z1", viscous=True)
+        s.set_initial(u0)
+        s.steps(1, dt)  # warm-up (graph capture)
+        s.set_initial(u0)
+        hip.synchronize()
+        e0.record(s.stream)
+        s.steps(len(dts) - 1, dt)
+        s.steps(1, dts[-1])
+        e1.record(s.stream)
+        e1.synchronize()
+        t_fused = e0.elapsed_ms(e1) / 1e3
+        fused = s.solution()
+        del s
+        npts = nx * ny * nz
+        case = {"grid": [nx, ny, nz], "steps": len(dts), "fused_s": t_fused,
+                "fused_gpts": npts * len(dts) / t_fused / 1e9,
+                "published_kernel_s": published_s, "published_steps": published_steps,
+                "published_gpts": npts * published_steps / published_s / 1e9,
+                "speedup_vs_published": (npts * len(dts) / t_fused) / (npts * published_steps / published_s),
+                "finite": bool(np.isfinite(fused).all())}
+        if dropin_steps:
+            # the drop-in sequence (5 launches and a copy per stage) on the same GPU, a few steps
+            R.set_adv_scheme()
+            du, duo, dLu = DeviceArray.from_host(u0), DeviceArray(u0.shape), DeviceArray(u0.shape)
+            dLu.zero()
+            R.burgers_reference_step(du, duo, dLu, nx, ny, nz, dx, dy, dz, dt)  # warm-up
+            hip.synchronize()
+            e0.record(None)
+            for _ in range(dropin_steps):
+                R.burgers_reference_step(du, duo, dLu, nx, ny, nz, dx, dy, dz, dt)
+            e1.record(None)
+            e1.synchronize()
+            t_drop = e0.elapsed_ms(e1) / 1e3
+            case.update({"dropin_steps": dropin_steps, "dropin_gpts": npts * dropin_steps / t_drop / 1e9,
+                         "fused_vs_dropin": (npts * len(dts) / t_fused) / (npts * dropin_steps / t_drop)})
+            del du, duo, dLu
+        out["cases"].append(case)
+    return out
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
-    ap.add_argument("study", choices=["batched", "fp32", "adjoint"])
+    ap.add_argument("study", choices=["batched", "fp32", "adjoint", "reference"])
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--n", type=int, default=512)
     args = ap.parse_args(argv)
     if args.study == "batched":
         r = batched(args.steps)
+    elif args.study == "reference":
+        r = reference_cases()
     elif args.study == "adjoint":
         r = adjoint(args.n if args.n != 512 else 128)
     else:
