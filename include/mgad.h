@@ -376,6 +376,16 @@ int mgad_check_division(hipStream_t stream, int mode, const double *a,
                         const double *b, double *q_fast, double *q_ieee,
                         size_t n);
 
+/* The reference's heat step fused (SingleGPU/Diffusion3d_baselineCode/
+ * main.cpp:108-121): the same u as nsteps times {uo <- u; for rk = 1..3:
+ * Call_Lu3d, Call_RK3d(rk)} with the CopyToConstantMemory coefficients, bit
+ * for bit, one launch per stage.  Interior nodes 2..n-3 are updated, the two
+ * ghost layers kept; w1, w2 are scratch fields (each call copies u into both).
+ * Requires n >= 5 per axis. */
+int mgad_heat13_rk3_steps(hipStream_t stream, unsigned int nx, unsigned int ny,
+                          unsigned int nz, double dt, double *u, double *w1,
+                          double *w2, int nsteps, int use_graph);
+
 /* ------------------------------------------------------------------------ */
 /* Host helpers (CPU only)                                                   */
 /* ------------------------------------------------------------------------ */

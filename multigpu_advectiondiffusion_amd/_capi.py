@@ -29,7 +29,7 @@ EXPORTS = (
     "mgad_burgers_init_p", "mgad_burgers_steps_p", "mgad_burgers_autotune", "mgad_burgers_set_launch",
     "mgad_batch_workspace_bytes", "mgad_batch_init", "mgad_batch_steps", "mgad_batch_steps_serial",
     "mgad_adjoint_workspace_bytes", "mgad_burgers_adjoint",
-    "mgad_burgers_ref_steps", "mgad_heat7_fe_steps", "mgad_check_division", "mgad_ic_fourier", "mgad_ic_gauss_noise",
+    "mgad_burgers_ref_steps", "mgad_heat7_fe_steps", "mgad_heat13_rk3_steps", "mgad_check_division", "mgad_ic_fourier", "mgad_ic_gauss_noise",
     "mgad_save_binary3d", "mgad_load_binary3d", "mgad_init3d_burgers", "mgad_init3d_heat", "mgad_heat_error_norms",
 )
 
@@ -148,6 +148,8 @@ def lib():
         L.mgad_heat7_fe_steps.restype = I
         L.mgad_burgers_ref_steps.argtypes = [VP, ctypes.POINTER(Scheme), I, U, U, U, D, D, D, D, VP, VP, VP, I, I]
         L.mgad_burgers_ref_steps.restype = I
+        L.mgad_heat13_rk3_steps.argtypes = [VP, U, U, U, D, VP, VP, VP, I, I]
+        L.mgad_heat13_rk3_steps.restype = I
         L.mgad_check_division.argtypes = [VP, I, VP, VP, VP, VP, ctypes.c_size_t]
         L.mgad_check_division.restype = I
         L.mgad_save_binary3d.argtypes = [VP, U, U, U, ctypes.c_char_p]

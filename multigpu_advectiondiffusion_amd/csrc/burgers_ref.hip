@@ -37,12 +37,6 @@ struct RefSmem {
     double FY[TY + 1][TX];
 };
 
-// 13-point second difference of the drop-ins, (-a + 16b - 30c + 16d - e) in
-// the source's left-to-right order; 16x is exact, so it folds into an FMA.
-__device__ __forceinline__ double d4(double a, double b, double c, double d, double e) {
-    return p2fma(16.0, d, p2fma(16.0, b, -a) - 30.0 * c) - e;
-}
-
 template <int WENO, int P, bool VISC, int TX, int TY>
 __global__ __launch_bounds__(TX * TY, 4) void k_burgers_ref_stage(const double *__restrict__ uin,
                                                                   const double *un, double *uout, RefArgs g,

@@ -33,7 +33,53 @@ __global__ __launch_bounds__(256) void k_heat7_fe(const double *__restrict__ u,
     v[o] = uo + dt * Lu;
 }
 
+// One SSP-RK3 stage of the reference heat step fused: the 13-point Laplacian
+// of Compute_Laplace3d (Diffusion3d_baselineCode/kernels.cu:94-97, with the
+// CopyToConstantMemory coefficients) and Compute_RK (kernels.cu:127-140) on
+// the interior 2..n-3 of every axis, Lu kept in a register.  Each thread
+// marches a chunk of one (x, y) column with a 5-plane queue of u for the z
+// neighbours; x and y neighbours come through the caches.
+__global__ __launch_bounds__(256) void k_heat13_rk(const double *__restrict__ uin, const double *un, double *uout,
+                                                   int nx, int ny, int nz, int kchunk, double kx, double ky,
+                                                   double kz, double dt, int stage) {
+    const int i = 2 + blockIdx.x * 64 + (threadIdx.x & 63);
+    const int j = 2 + blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int z0 = 2 + blockIdx.z * kchunk, z1 = min(z0 + kchunk, nz - 2);
+    if (i >= nx - 2 || j >= ny - 2) return;
+    const unsigned xy = static_cast<unsigned>(nx) * ny, col = i + static_cast<unsigned>(nx) * j;
+    const unsigned px = nx, px2 = 2u * nx;
+    double q[5];  // u at planes k-2..k+2
+#pragma unroll
+    for (int t = 0; t < 4; t++) q[t] = uin[col + xy * static_cast<unsigned>(z0 - 2 + t)];
+    for (int k = z0; k < z1; k++) {
+        const unsigned o = col + xy * static_cast<unsigned>(k);
+        q[4] = uin[o + 2u * xy];
+        const double c = q[2];
+        const double X = d4(uin[o + 2], uin[o + 1], c, uin[o - 1], uin[o - 2]);
+        const double Y = d4(uin[o + px2], uin[o + px], c, uin[o - px], uin[o - px2]);
+        const double Z = d4(q[4], q[3], c, q[1], q[0]);
+        const double Lu = (kx * X + ky * Y) + kz * Z;
+        const double uo = stage == 1 ? c : un[o];
+        uout[o] = rk_stage(stage, dt, c, uo, Lu);
+#pragma unroll
+        for (int t = 0; t < 4; t++) q[t] = q[t + 1];
+    }
+}
+
 namespace {
+
+int enqueue_heat13(hipStream_t s, unsigned nx, unsigned ny, unsigned nz, DiffCoef c, double dt, double *u,
+                   double *w1, double *w2) {
+    const int kchunk = 16;
+    const dim3 grid((nx - 4 + 63) / 64, (ny - 4 + 3) / 4, (nz - 4 + kchunk - 1) / kchunk);
+    hipLaunchKernelGGL(k_heat13_rk, grid, dim3(256), 0, s, u, u, w1, (int)nx, (int)ny, (int)nz, kchunk, c.kx, c.ky,
+                       c.kz, dt, 1);
+    hipLaunchKernelGGL(k_heat13_rk, grid, dim3(256), 0, s, w1, u, w2, (int)nx, (int)ny, (int)nz, kchunk, c.kx, c.ky,
+                       c.kz, dt, 2);
+    hipLaunchKernelGGL(k_heat13_rk, grid, dim3(256), 0, s, w2, u, u, (int)nx, (int)ny, (int)nz, kchunk, c.kx, c.ky,
+                       c.kz, dt, 3);
+    return check_launch();
+}
 
 int enqueue_heat(hipStream_t s, unsigned nx, unsigned ny, unsigned nz, double kx, double ky,
                  double kz, double dt, double *u, double *w, int nsteps) {
@@ -106,4 +152,54 @@ extern "C" int mgad_heat7_fe_steps(hipStream_t stream, unsigned int nx, unsigned
         exec = it->second;
     }
     return check_hip(hipGraphLaunch(exec, stream));
+}
+
+// The reference's heat step fused (SingleGPU/Diffusion3d_baselineCode/
+// main.cpp:108-121): per step uo <- u, then for rk = 1..3 Call_Lu3d and
+// Call_RK3d(rk), one launch per stage; the two ghost layers are not written.
+extern "C" int mgad_heat13_rk3_steps(hipStream_t stream, unsigned int nx, unsigned int ny, unsigned int nz, double dt,
+                                     double *u, double *w1, double *w2, int nsteps, int use_graph) {
+    if (nx < 5 || ny < 5 || nz < 5 || 1ull * nx * ny * nz >= (1ull << 31) || !u || !w1 || !w2 || u == w1 ||
+        u == w2 || w1 == w2 || nsteps < 0) {
+        set_error(MGAD_ERR_INVALID_ARG);
+        return MGAD_ERR_INVALID_ARG;
+    }
+    const size_t bytes = sizeof(double) * nx * ny * nz;
+    int st = check_hip(hipMemcpyAsync(w1, u, bytes, hipMemcpyDeviceToDevice, stream));
+    if (!st) st = check_hip(hipMemcpyAsync(w2, u, bytes, hipMemcpyDeviceToDevice, stream));
+    if (st || nsteps == 0) return st;
+    const DiffCoef c = diff_coef();
+    if (!use_graph || stream == nullptr) {
+        for (int s = 0; s < nsteps && !st; s++) st = enqueue_heat13(stream, nx, ny, nz, c, dt, u, w1, w2);
+        return st;
+    }
+    using Key = std::tuple<hipStream_t, unsigned, unsigned, unsigned, double, double, double, double, double *,
+                           double *, double *>;
+    static std::map<Key, hipGraphExec_t> cache;
+    const Key key{stream, nx, ny, nz, c.kx, c.ky, c.kz, dt, u, w1, w2};
+    hipGraphExec_t exec = nullptr;
+    auto it = cache.find(key);
+    if (it == cache.end()) {
+        if (cache.size() > 16) {
+            for (auto &kv : cache) (void)hipGraphExecDestroy(kv.second);
+            cache.clear();
+        }
+        hipGraph_t g = nullptr;
+        if ((st = check_hip(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal)))) return st;
+        const int est = enqueue_heat13(stream, nx, ny, nz, c, dt, u, w1, w2);
+        st = check_hip(hipStreamEndCapture(stream, &g));
+        if (est) {
+            if (g) (void)hipGraphDestroy(g);
+            return est;
+        }
+        if (st) return st;
+        st = check_hip(hipGraphInstantiate(&exec, g, nullptr, nullptr, 0));
+        (void)hipGraphDestroy(g);
+        if (st) return st;
+        cache[key] = exec;
+    } else {
+        exec = it->second;
+    }
+    for (int s = 0; s < nsteps && !st; s++) st = check_hip(hipGraphLaunch(exec, stream));
+    return st;
 }

@@ -272,6 +272,14 @@ __device__ __forceinline__ void lf_split_local(double u, double &fp, double &fm)
     fm = 0.5 * (f - au);
 }
 
+// Fourth-order second difference of the 13-point Laplacians,
+// (-a + 16b - 30c + 16d - e) in the source's left-to-right order
+// (kernels.cu:474-476, Diffusion3d_baselineCode/kernels.cu:94-97); 16x is
+// exact, so it folds into an FMA.
+__device__ __forceinline__ double d4(double a, double b, double c, double d, double e) {
+    return p2fma(16.0, d, p2fma(16.0, b, -a) - 30.0 * c) - e;
+}
+
 // SSP-RK3 stage at one node, kernels.cu:520-527.
 __device__ __forceinline__ double rk_stage(int step, double dt, double u, double uo, double Lu) {
     if (step == 1) return uo + dt * Lu;

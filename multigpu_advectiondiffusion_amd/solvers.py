@@ -265,6 +265,31 @@ class ReferenceBurgers:
         return self.u.download()
 
 
+class ReferenceHeat:
+    """The reference driver's heat step (SingleGPU/Diffusion3d_baselineCode/
+    main.cpp:108-121: 13-point Laplacian with the CopyToConstantMemory
+    coefficients and SSP-RK3 on the interior, two frozen ghost layers) as one
+    fused launch per stage."""
+
+    def __init__(self, shape, stream: Stream | None = None):
+        self.nx, self.ny, self.nz = (shape, shape, shape) if np.isscalar(shape) else shape
+        self.stream = stream if stream is not None else Stream()
+        shp = (self.nz, self.ny, self.nx)
+        self.u, self.w1, self.w2 = DeviceArray(shp), DeviceArray(shp), DeviceArray(shp)
+
+    def set_initial(self, u0: np.ndarray) -> None:
+        self.u.upload(np.ascontiguousarray(u0, dtype=np.float64))
+
+    def steps(self, nsteps: int, dt: float, graph: bool = True) -> None:
+        rc = _capi.lib().mgad_heat13_rk3_steps(self.stream.handle, self.nx, self.ny, self.nz, dt, self.u.ptr,
+                                               self.w1.ptr, self.w2.ptr, nsteps, 1 if graph else 0)
+        _capi.check(rc, "mgad_heat13_rk3_steps")
+
+    def solution(self) -> np.ndarray:
+        self.stream.synchronize()
+        return self.u.download()
+
+
 def heat7_params(shape, C: float = 1.0, L: float = 1.0, r: float = 0.15) -> dict:
     """Node-inclusive spacing dx = L/(n-1) (SingleGPU/Diffusion3d_baselineCode/
     main.cpp:55-57), coefficients kx = C/(12 dx^2) (:59-61), dt = r dx^2 / C."""

@@ -4,6 +4,7 @@
     python -m multigpu_advectiondiffusion_amd.studies fp32      [--n 512]
     python -m multigpu_advectiondiffusion_amd.studies adjoint   [--n 128]
     python -m multigpu_advectiondiffusion_amd.studies reference
+    python -m multigpu_advectiondiffusion_amd.studies heat
 
 batched: 64 independent periodic WENO5-Z (p=2, eps=1e-40) FP64 problems with
     edge lengths drawn seeded-uniformly from {32,48,64,96,128} (SplitMix64 seed
@@ -288,9 +289,68 @@ def reference_cases(cases=REFERENCE_CASES, dropin_steps: int = 3) -> dict:
     return out
 
 
+def heat_cases(dropin_steps: int = 5) -> dict:
+    """Heat: the 7-point forward-Euler step (64^3, 100 steps, the parity
+    configuration; 512^3, HBM-sized) and the reference's own 13-point SSP-RK3
+    run (SingleGPU/Diffusion3d_baselineCode/Run.m:1-25: 201x199x200, C = 1,
+    L = 10, 605 steps of dt = 0.9/(2C sum 1/dx^2), Call_Init3D ic 3 at
+    t0 = 0.1) fused, against its published kernel seconds, with the
+    corrected CalcError norms at the final time."""
+    from . import fileio
+    from . import reference_api as R
+    from .hip import DeviceArray
+
+    out = {"study": "heat"}
+    e0, e1 = hip.Event(), hip.Event()
+
+    def timed(stream, fn):
+        hip.synchronize()
+        e0.record(stream)
+        fn()
+        e1.record(stream)
+        e1.synchronize()
+        return e0.elapsed_ms(e1) / 1e3
+
+    for n, steps in ((64, 100), (512, 20)):
+        h = solvers.Heat7(n)
+        h.set_initial(solvers.ic_gauss_noise(n, seed=1))
+        h.steps(steps)  # warm-up (graph capture)
+        t = timed(h.stream, lambda: h.steps(steps))
+        gbs = 16.0 * n ** 3 * steps / t / 1e9
+        out[f"heat7_fe_{n}"] = {"steps": steps, "gpts": n ** 3 * steps / t / 1e9, "algorithmic_GBs": gbs,
+                                "hbm_frac": gbs / 8000.0}
+        del h
+    nx, ny, nz, C, L, iters, published_s = 201, 199, 200, 1.0, 10.0, 605, 5.81
+    dx, dy, dz = L / (nx - 1), L / (ny - 1), L / (nz - 1)
+    dt = 1 / (2 * C * (1 / dx / dx + 1 / dy / dy + 1 / dz / dz)) * 0.9
+    kc = (C / (12 * dx * dx), C / (12 * dy * dy), C / (12 * dz * dz))
+    R.CopyToConstantMemory(*kc)
+    u0 = fileio.init3d_heat(3, (nx, ny, nz), dx, dy, dz)
+    h = solvers.ReferenceHeat((nx, ny, nz))
+    h.set_initial(u0)
+    h.steps(1, dt)  # warm-up
+    h.set_initial(u0)
+    t = timed(h.stream, lambda: h.steps(iters, dt))
+    u = h.solution()
+    npts = nx * ny * nz
+    l1, l2, linf = fileio.heat_error_norms(u, 0.1 + dt * iters, dx, dy, dz)
+    case = {"grid": [nx, ny, nz], "steps": iters, "fused_s": t, "fused_gpts": npts * iters / t / 1e9,
+            "algorithmic_GBs": 64.0 * npts * iters / t / 1e9, "published_kernel_s": published_s,
+            "published_gpts": npts * iters / published_s / 1e9, "speedup_vs_published": published_s / t,
+            "final_time": 0.1 + dt * iters, "error_l1": l1, "error_l2": l2, "error_linf": linf}
+    du, duo, dLu = DeviceArray.from_host(u0), DeviceArray(u0.shape), DeviceArray(u0.shape)
+    dLu.zero()
+    R.heat_reference_step(du, duo, dLu, nx, ny, nz, dt)  # warm-up
+    td = timed(None, lambda: [R.heat_reference_step(du, duo, dLu, nx, ny, nz, dt) for _ in range(dropin_steps)])
+    case.update({"dropin_gpts": npts * dropin_steps / td / 1e9,
+                 "fused_vs_dropin": (npts * iters / t) / (npts * dropin_steps / td)})
+    out["reference_heat13_rk3"] = case
+    return out
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
-    ap.add_argument("study", choices=["batched", "fp32", "adjoint", "reference"])
+    ap.add_argument("study", choices=["batched", "fp32", "adjoint", "reference", "heat"])
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--n", type=int, default=512)
     args = ap.parse_args(argv)
@@ -298,6 +358,8 @@ def main(argv=None) -> int:
         r = batched(args.steps)
     elif args.study == "reference":
         r = reference_cases()
+    elif args.study == "heat":
+        r = heat_cases()
     elif args.study == "adjoint":
         r = adjoint(args.n if args.n != 512 else 128)
     else:

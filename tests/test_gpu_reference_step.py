@@ -74,3 +74,28 @@ def test_fused_reference_step_matches_dropins_on_gpu(hipdev):
 def test_fused_reference_step_rejects_small_grid(hipdev):
     with pytest.raises(Exception):
         solvers.ReferenceBurgers((6, 8, 8), 0.1, 0.1, 0.1).steps(1, 0.01)
+
+
+@pytest.mark.parametrize("shape", [(24, 20, 18), (5, 6, 7), (67, 9, 35)])
+@pytest.mark.parametrize("graph", [False, True])
+def test_fused_reference_heat_bitwise(hipdev, shape, graph):
+    """mgad_heat13_rk3_steps against the oracle's Call_Lu3d + Call_RK3d sequence."""
+    nx, ny, nz = shape
+    L = 10.0
+    dx, dy, dz = L / (nx - 1), L / (ny - 1), L / (nz - 1)
+    kc = (1.0 / (12 * dx * dx), 1.0 / (12 * dy * dy), 1.0 / (12 * dz * dz))
+    R.CopyToConstantMemory(*kc)
+    dt = 1.0 / (2 * (1 / dx / dx + 1 / dy / dy + 1 / dz / dz)) * 0.9
+    u0 = fileio.init3d_heat(3, shape, dx, dy, dz) + 1e-3 * np.random.default_rng(3).standard_normal((nz, ny, nx))
+    h = solvers.ReferenceHeat(shape)
+    h.set_initial(u0)
+    h.steps(3, dt, graph=graph)
+    h.steps(2, dt, graph=graph)
+    u, Lu = u0.copy(), np.zeros_like(u0)
+    for _ in range(5):
+        uo = u.copy()
+        for step in (1, 2, 3):
+            O.lu3d_13(u, Lu, *kc)
+            O.rk3d(u, uo, Lu, step, dt)
+    got = h.solution()
+    assert np.array_equal(got, u), np.argwhere(got != u)[:5]
