@@ -356,7 +356,8 @@ int mgad_burgers_ref_steps(hipStream_t stream, const mgad_scheme *scheme,
 /* nsteps forward-Euler steps u <- u + dt*L7(u) on the Dirichlet interior
  * 0<i<n-1 (boundary nodes held), L7 as mgad_lu3d_o2 with coefficients
  * kx, ky, kz (= C/(12 dx^2)).  u and w are two device fields; the result is
- * left in u.  Asynchronous on stream. */
+ * left in u.  Requires n >= 3 per axis, nx*ny*nz < 2^31 and nx*ny < 2^29.
+ * Asynchronous on stream. */
 int mgad_heat7_fe_steps(hipStream_t stream, unsigned int nx, unsigned int ny,
                         unsigned int nz, double kx, double ky, double kz,
                         double dt, double *u, double *w, int nsteps,
@@ -381,7 +382,7 @@ int mgad_check_division(hipStream_t stream, int mode, const double *a,
  * Call_Lu3d, Call_RK3d(rk)} with the CopyToConstantMemory coefficients, bit
  * for bit, one launch per stage.  Interior nodes 2..n-3 are updated, the two
  * ghost layers kept; w1, w2 are scratch fields (each call copies u into both).
- * Requires n >= 5 per axis. */
+ * Requires n >= 5 per axis, nx*ny*nz < 2^31 and nx*ny < 2^29. */
 int mgad_heat13_rk3_steps(hipStream_t stream, unsigned int nx, unsigned int ny,
                           unsigned int nz, double dt, double *u, double *w1,
                           double *w2, int nsteps, int use_graph);

@@ -37,13 +37,6 @@ struct GridArgs {
 
 __device__ __forceinline__ int wrapi(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
 
-// Maps the launch's linear block id to a tile so that the blocks one XCD
-// runs (ids congruent mod 8) are contiguous tiles sharing halos in L2.
-__device__ __forceinline__ unsigned xcd_tile(unsigned bid, unsigned nblocks) {
-    if ((nblocks & 7u) != 0u) return bid;
-    return (bid & 7u) * (nblocks >> 3) + (bid >> 3);
-}
-
 // Step control shared by every block of a stage.  Stage 1 derives dt from
 // max|u^n| (all blocks compute the same value) and records it; stage 2 moves
 // t forward.  Each stage's lead thread clears the max|u| slot the next-but-one

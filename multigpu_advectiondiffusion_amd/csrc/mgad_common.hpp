@@ -287,6 +287,13 @@ __device__ __forceinline__ double rk_stage(int step, double dt, double u, double
     return p2fma(2.0, u + dt * Lu, uo) / 3.0;  // (uo + 2(u + dt Lu)) / 3
 }
 
+// Maps the launch's linear block id to a tile so that the blocks one XCD
+// runs (ids congruent mod 8) are contiguous tiles sharing halos in L2.
+__device__ __forceinline__ unsigned xcd_tile(unsigned bid, unsigned nblocks) {
+    if ((nblocks & 7u) != 0u) return bid;
+    return (bid & 7u) * (nblocks >> 3) + (bid >> 3);
+}
+
 // Non-negative doubles order like their bit patterns as unsigned integers, so
 // a max over |u| is an integer atomicMax; NaN bit patterns sort above +Inf.
 __device__ __forceinline__ unsigned long long abs_bits(double x) {

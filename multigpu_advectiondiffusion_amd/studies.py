@@ -295,7 +295,9 @@ def heat_cases(dropin_steps: int = 5) -> dict:
     run (SingleGPU/Diffusion3d_baselineCode/Run.m:1-25: 201x199x200, C = 1,
     L = 10, 605 steps of dt = 0.9/(2C sum 1/dx^2), Call_Init3D ic 3 at
     t0 = 0.1) fused, against its published kernel seconds, with the
-    corrected CalcError norms at the final time."""
+    corrected CalcError norms at the final time, and the same step at 512^3.
+    A plain device copy of a 512^3 field gives the bandwidth a streaming
+    kernel can reach."""
     from . import fileio
     from . import reference_api as R
     from .hip import DeviceArray
@@ -311,6 +313,12 @@ def heat_cases(dropin_steps: int = 5) -> dict:
         e1.synchronize()
         return e0.elapsed_ms(e1) / 1e3
 
+    # what a plain device-to-device copy of one 512^3 field moves (read + write)
+    a, b = DeviceArray((512, 512, 512)), DeviceArray((512, 512, 512))
+    b.copy_from(a)
+    t = timed(None, lambda: [b.copy_from(a) for _ in range(10)])
+    out["device_copy_512"] = {"GBs": 2 * 8.0 * 512 ** 3 * 10 / t / 1e9}
+    del a, b
     for n, steps in ((64, 100), (512, 20)):
         h = solvers.Heat7(n)
         h.set_initial(solvers.ic_gauss_noise(n, seed=1))
@@ -345,6 +353,17 @@ def heat_cases(dropin_steps: int = 5) -> dict:
     case.update({"dropin_gpts": npts * dropin_steps / td / 1e9,
                  "fused_vs_dropin": (npts * iters / t) / (npts * dropin_steps / td)})
     out["reference_heat13_rk3"] = case
+    del h, du, duo, dLu
+    # the same fused step on a 512^3 grid (bandwidth once launches are long;
+    # the timed call includes its two initial field copies, ~2 %)
+    n, steps = 512, 20
+    h = solvers.ReferenceHeat((n, n, n))
+    h.set_initial(solvers.ic_gauss_noise(n, seed=1))
+    h.steps(1, 1e-6)  # warm-up (graph capture)
+    t = timed(h.stream, lambda: h.steps(steps, 1e-6))
+    out["heat13_rk3_512"] = {"steps": steps, "gpts": n ** 3 * steps / t / 1e9,
+                             "algorithmic_GBs": 64.0 * n ** 3 * steps / t / 1e9,
+                             "hbm_frac": 64.0 * n ** 3 * steps / t / 1e9 / 8000.0}
     return out
 
 

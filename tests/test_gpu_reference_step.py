@@ -76,7 +76,10 @@ def test_fused_reference_step_rejects_small_grid(hipdev):
         solvers.ReferenceBurgers((6, 8, 8), 0.1, 0.1, 0.1).steps(1, 0.01)
 
 
-@pytest.mark.parametrize("shape", [(24, 20, 18), (5, 6, 7), (67, 9, 35)])
+# (67, 9, 35): z chunks of 8 planes and a remainder of 7; (1024, 516, 45): one
+# chunk of 41 planes, five passes of the kernel's unrolled plane loop and one
+# more plane
+@pytest.mark.parametrize("shape", [(24, 20, 18), (5, 6, 7), (67, 9, 35), (1024, 516, 45)])
 @pytest.mark.parametrize("graph", [False, True])
 def test_fused_reference_heat_bitwise(hipdev, shape, graph):
     """mgad_heat13_rk3_steps against the oracle's Call_Lu3d + Call_RK3d sequence."""
