@@ -295,6 +295,46 @@ int mgad_batch_steps_serial(hipStream_t stream, const mgad_batch *batch,
                             int use_graph);
 
 /* ------------------------------------------------------------------------ */
+/* z-slab decomposition of the periodic Burgers run (FP64)                   */
+/* ------------------------------------------------------------------------ */
+
+/* The layout and halo exchange of the z-slab decomposition of
+ * MultiGPU/Burgers3d_Baseline (main.c:64-66, 3 ghost layers BurgersMPI.h:50),
+ * with the fused stage kernel on each slab.  A slab set holds nslab slabs of
+ * nzs planes of a periodic nx x ny x (nslab*nzs) grid one after another;
+ * slab s stores nzs + 6 planes: 3 ghost planes, its own planes
+ * s*nzs .. s*nzs+nzs-1, 3 ghost planes (nx*ny*(nzs+6) doubles per slab).
+ * Requires nzs >= 3 and nx*ny*(nzs+6) < 2^31. */
+
+/* Ghost planes of every slab from its neighbours' boundary planes (slab
+ * indices periodic; one slab wraps onto itself).  Each side is 3 contiguous
+ * planes, the data one GPU of the decomposition would send to and receive
+ * from each neighbour per stage.  Replaces the CopyBoundaryRegionToGhostCell
+ * + MPI + CopyGhostCellToBoundaryRegion sequence of MultiGPU/
+ * Burgers3d_Baseline/main.c:207-294.  Asynchronous on stream. */
+int mgad_slab_exchange(hipStream_t stream, unsigned int nx, unsigned int ny,
+                       unsigned int nzs, unsigned int nslab, double *u);
+
+/* mgad_burgers_init for a slab set: exchanges u's ghosts, then max|u| over
+ * the own planes of all slabs.  ctrl: MGAD_CTRL_BYTES of device memory. */
+int mgad_burgers_slab_init(hipStream_t stream, unsigned int nx,
+                           unsigned int ny, unsigned int nzs,
+                           unsigned int nslab, double *u, void *ctrl,
+                           double t0, double tend, double cfl);
+
+/* nsteps SSP-RK3 steps of the periodic run on a slab set: per stage one
+ * launch over all slabs (one control block: max|u| and dt are global) and
+ * one exchange of the stage's output.  u, w1, w2: slab sets of the same
+ * shape; nx, ny multiples of 16.  The own planes equal mgad_burgers_steps
+ * on the unsplit grid bit for bit.  Asynchronous on stream. */
+int mgad_burgers_slab_steps(hipStream_t stream, const mgad_scheme *scheme,
+                            unsigned int nx, unsigned int ny,
+                            unsigned int nzs, unsigned int nslab, double dx,
+                            double dy, double dz, double *u, double *w1,
+                            double *w2, void *ctrl, int nsteps,
+                            int use_graph);
+
+/* ------------------------------------------------------------------------ */
 /* Discrete adjoint of the periodic Burgers run (FP64)                       */
 /* ------------------------------------------------------------------------ */
 

@@ -31,6 +31,7 @@ EXPORTS = (
     "mgad_adjoint_workspace_bytes", "mgad_burgers_adjoint",
     "mgad_burgers_ref_steps", "mgad_heat7_fe_steps", "mgad_heat13_rk3_steps", "mgad_check_division", "mgad_ic_fourier", "mgad_ic_gauss_noise",
     "mgad_save_binary3d", "mgad_load_binary3d", "mgad_init3d_burgers", "mgad_init3d_heat", "mgad_heat_error_norms",
+    "mgad_slab_exchange", "mgad_burgers_slab_init", "mgad_burgers_slab_steps",
 )
 
 
@@ -150,6 +151,12 @@ def lib():
         L.mgad_burgers_ref_steps.restype = I
         L.mgad_heat13_rk3_steps.argtypes = [VP, U, U, U, D, VP, VP, VP, I, I]
         L.mgad_heat13_rk3_steps.restype = I
+        L.mgad_slab_exchange.argtypes = [VP, U, U, U, U, VP]
+        L.mgad_slab_exchange.restype = I
+        L.mgad_burgers_slab_init.argtypes = [VP, U, U, U, U, VP, VP, D, D, D]
+        L.mgad_burgers_slab_init.restype = I
+        L.mgad_burgers_slab_steps.argtypes = [VP, ctypes.POINTER(Scheme), U, U, U, U, D, D, D, VP, VP, VP, VP, I, I]
+        L.mgad_burgers_slab_steps.restype = I
         L.mgad_check_division.argtypes = [VP, I, VP, VP, VP, VP, ctypes.c_size_t]
         L.mgad_check_division.restype = I
         L.mgad_save_binary3d.argtypes = [VP, U, U, U, ctypes.c_char_p]

@@ -99,7 +99,10 @@ struct TileSmem {
 
 // One tile x z-chunk of one stage; ends with one atomicMax per block into the
 // stage's max|u| slot.  FD: fast_division_ok<WENO>(alpha, eps) holds.
-template <class T, int WENO, int P, int TX, int TY, bool FD>
+// ZG: the field is one z slab of nz planes stored between 3 ghost planes
+// below and 3 above (plane k at storage plane k + 3, read for k in -3..nz+2,
+// written for k in 0..nz-1); otherwise z is periodic over nz planes.
+template <class T, int WENO, int P, int TX, int TY, bool FD, bool ZG = false>
 __device__ __forceinline__ void stage_tile(const T *__restrict__ uin, const T *un, T *uout,
                                            const GridArgs &g, int stage, unsigned tile, double dt,
                                            double alpha, double eps, TileSmem<TX, TY> &sm,
@@ -122,6 +125,9 @@ __device__ __forceinline__ void stage_tile(const T *__restrict__ uin, const T *u
     const int wave = tid >> 6, lane = tid & 63;
     const unsigned plane = static_cast<unsigned>(nx) * static_cast<unsigned>(ny);
     const unsigned col = static_cast<unsigned>(x0 + tx) + static_cast<unsigned>(nx) * (y0 + ty);
+    // storage plane of plane k: zw for k in -3..nz+3 (wrapped when periodic), zi for 0 <= k < nz
+    auto zw = [&](int k) { return static_cast<unsigned>(ZG ? k + 3 : wrapi(k, nz)); };
+    auto zi = [&](int k) { return static_cast<unsigned>(ZG ? k + 3 : k); };
 
     // Halo node handled by this thread: 6*TY x-halo nodes, then 6*TX y-halo nodes.
     int hl_r = 0, hl_c = 0;
@@ -145,7 +151,7 @@ __device__ __forceinline__ void stage_tile(const T *__restrict__ uin, const T *u
     double qp[7], qm[7], sp[4], sn[4], qu[4];
 #pragma unroll
     for (int q = 0; q < 6; q++) {
-        const double v = ld(uin, col + plane * static_cast<unsigned>(wrapi(z0 - 3 + q, nz)));
+        const double v = ld(uin, col + plane * zw(z0 - 3 + q));
         lf_split(v, alpha, qp[q], qm[q]);
         if (q >= 3) qu[q - 3] = v;
     }
@@ -164,19 +170,20 @@ __device__ __forceinline__ void stage_tile(const T *__restrict__ uin, const T *u
     const T *un_ptr = un + col;
     T *out_ptr = uout + col;
     const T *halo_ptr = uin + hcol;
-    int kn = z0 + 3 >= nz ? z0 + 3 - nz : z0 + 3;  // plane of the next own-column load
-    double own_next = ld(own_ptr, plane * static_cast<unsigned>(kn));
-    double halo_next = has_halo ? ld(halo_ptr, plane * static_cast<unsigned>(z0)) : 0.0;
-    double un_next = (stage == 1) ? 0.0 : ld(un_ptr, plane * static_cast<unsigned>(z0));
+    // storage plane of the next own-column load (plane k + 3)
+    unsigned kn = zw(z0 + 3);
+    double own_next = ld(own_ptr, plane * kn);
+    double halo_next = has_halo ? ld(halo_ptr, plane * zi(z0)) : 0.0;
+    double un_next = (stage == 1) ? 0.0 : ld(un_ptr, plane * zi(z0));
     unsigned long long vmax = 0ull;
 
     for (int k = z0; k < z1; k++) {
         const double own = own_next, halo = halo_next, unv = un_next;
         if (k + 1 < z1) {
-            kn = kn + 1 == nz ? 0 : kn + 1;
-            own_next = ld(own_ptr, plane * static_cast<unsigned>(kn));
-            if (has_halo) halo_next = ld(halo_ptr, plane * static_cast<unsigned>(k + 1));
-            if (stage != 1) un_next = ld(un_ptr, plane * static_cast<unsigned>(k + 1));
+            kn = (!ZG && kn + 1 == static_cast<unsigned>(nz)) ? 0u : kn + 1;
+            own_next = ld(own_ptr, plane * kn);
+            if (has_halo) halo_next = ld(halo_ptr, plane * zi(k + 1));
+            if (stage != 1) un_next = ld(un_ptr, plane * zi(k + 1));
         }
         lf_split(own, alpha, qp[6], qm[6]);
         qu[3] = own;
@@ -239,7 +246,7 @@ __device__ __forceinline__ void stage_tile(const T *__restrict__ uin, const T *u
         else
             unew = div_by(p2fma(2.0, qu[0] + dt * Lu, uo), 3.0, 1.0 / 3.0);  // (uo + 2(..)) / 3
         const T stored = static_cast<T>(unew);
-        out_ptr[plane * static_cast<unsigned>(k)] = stored;
+        out_ptr[plane * zi(k)] = stored;
         const unsigned long long b = abs_bits(static_cast<double>(stored));
         vmax = b > vmax ? b : vmax;
 

@@ -138,6 +138,52 @@ def set_launch(scheme: str, shape, tile: int = 0, kchunk: int = 0, precision: st
     _capi.check(rc, "mgad_burgers_set_launch")
 
 
+class SlabBurgers:
+    """PeriodicBurgers on z slabs with 3 ghost planes each (mgad_burgers_slab_*):
+    the data layout and per-stage halo exchange of the z-slab decomposition
+    (MultiGPU/Burgers3d_Baseline/main.c:64-66), all slabs on one device.  The
+    own planes equal PeriodicBurgers' field bit for bit."""
+
+    def __init__(self, shape, nslab: int, scheme: str = "Z2", cfl: float = 0.4, lo: float = -1.0,
+                 hi: float = 1.0, stream: Stream | None = None):
+        self.nx, self.ny, self.nz = (shape, shape, shape) if np.isscalar(shape) else shape
+        if self.nz % nslab:
+            raise ValueError(f"nz={self.nz} is not a multiple of nslab={nslab}")
+        self.nslab, self.nzs = nslab, self.nz // nslab
+        self.dx, self.dy, self.dz = (hi - lo) / self.nx, (hi - lo) / self.ny, (hi - lo) / self.nz
+        self.cfl = cfl
+        self.scheme = scheme_struct(scheme)
+        self.stream = stream if stream is not None else Stream()
+        shp = (self.nslab, self.nzs + 6, self.ny, self.nx)
+        self.u, self.w1, self.w2 = DeviceArray(shp), DeviceArray(shp), DeviceArray(shp)
+        self.ctrl = DeviceArray((_capi.CTRL_BYTES,), np.uint8)
+
+    def set_initial(self, u0: np.ndarray, t0: float = 0.0, tend: float = float("inf")) -> None:
+        """Scatter u0 (nz, ny, nx) into the slabs' own planes; the ghosts are
+        filled on the device by the exchange in mgad_burgers_slab_init."""
+        slabs = np.zeros((self.nslab, self.nzs + 6, self.ny, self.nx))
+        slabs[:, 3:3 + self.nzs] = np.asarray(u0, dtype=np.float64).reshape(self.nslab, self.nzs, self.ny, self.nx)
+        self.u.upload(slabs)
+        rc = _capi.lib().mgad_burgers_slab_init(self.stream.handle, self.nx, self.ny, self.nzs, self.nslab,
+                                                self.u.ptr, self.ctrl.ptr, t0, tend, self.cfl)
+        _capi.check(rc, "mgad_burgers_slab_init")
+
+    def steps(self, nsteps: int, graph: bool = True) -> None:
+        rc = _capi.lib().mgad_burgers_slab_steps(self.stream.handle, ctypes.byref(self.scheme), self.nx, self.ny,
+                                                 self.nzs, self.nslab, self.dx, self.dy, self.dz, self.u.ptr,
+                                                 self.w1.ptr, self.w2.ptr, self.ctrl.ptr, nsteps, 1 if graph else 0)
+        _capi.check(rc, "mgad_burgers_slab_steps")
+
+    def slabs(self) -> np.ndarray:
+        """The whole slab set, ghosts included: (nslab, nzs + 6, ny, nx)."""
+        self.stream.synchronize()
+        return self.u.download()
+
+    def solution(self) -> np.ndarray:
+        """The own planes gathered into (nz, ny, nx)."""
+        return self.slabs()[:, 3:3 + self.nzs].reshape(self.nz, self.ny, self.nx)
+
+
 class BurgersBatch:
     """Independent periodic Burgers problems of uneven cubic sizes advanced
     together (one persistent launch per SSP-RK3 stage), each with its own dt."""
