@@ -12,9 +12,11 @@ After the timed region (and the CPU baseline) it also runs the three other
 measurements the metric names, reported as extra fields: the batched
 uneven-size speedup over one-by-one launches (64 problems, 200 steps), the
 adjoint cost ratio and finite-difference agreement (128^3, 400 steps / 16
-directions), the FP32-storage error against FP64 (512^3 to t = 0.5), and the
+directions), the FP32-storage error against FP64 (512^3 to t = 0.5), the
 reference's own viscous Burgers runs with the fused reference step against
-the kernel times the reference published for them;
+the kernel times the reference published for them, and the z-slab form of
+the run (one 256x256x32 slab, one GPU's share of an 8-way 256^3
+decomposition; 8 slabs on one GPU) against the unsplit grid;
 --quick skips them.  Progress goes to stderr.
 """
 from __future__ import annotations
@@ -85,7 +87,8 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
-    ap.add_argument("--quick", action="store_true", help="skip the batched / adjoint / fp32 studies")
+    ap.add_argument("--quick", action="store_true",
+                    help="skip the batched / adjoint / reference / fp32 / slab studies")
     ap.add_argument("--copy-calib", action="store_true",
                     help="after timing, run 3 device copies of one field (counter calibration)")
     args = ap.parse_args()
@@ -192,6 +195,10 @@ def main() -> int:
         f = studies.fp32(n=512)
         out["fp32_storage_512"] = {k: f[k] for k in ("l1", "linf", "steps_f64", "gpts_f64", "gpts_f32")}
         print("fp32 study done", file=sys.stderr, flush=True)
+        sl = studies.slab_cases()
+        out["z_slabs"] = {k: sl[k] for k in ("one_slab_256x256x32_gpts", "eight_slabs_256_gpts", "unsplit_256_gpts",
+                                             "one_slab_vs_unsplit", "eight_slabs_vs_unsplit")}
+        print("slab study done", file=sys.stderr, flush=True)
     print(json.dumps(out), flush=True)
     return 0
 

@@ -5,6 +5,7 @@
     python -m multigpu_advectiondiffusion_amd.studies adjoint   [--n 128]
     python -m multigpu_advectiondiffusion_amd.studies reference
     python -m multigpu_advectiondiffusion_amd.studies heat
+    python -m multigpu_advectiondiffusion_amd.studies slab
 
 batched: 64 independent periodic WENO5-Z (p=2, eps=1e-40) FP64 problems with
     edge lengths drawn seeded-uniformly from {32,48,64,96,128} (SplitMix64 seed
@@ -31,6 +32,12 @@ reference: the reference's own viscous Burgers runs (512^3 and 1000x1000x200
     to tEnd 0.1, 1601x986x35 to 0.4) with the fused reference-step kernel, against
     the kernel seconds the reference published for them, plus the drop-in
     sequence on the same GPU for a few steps.
+slab: the headline WENO5-Z (p=2) FP64 run on z slabs with 3 ghost planes:
+    one GPU's share of an 8-way 256^3 decomposition (one 256x256x32 slab
+    whose exchange wraps onto itself) and all 8 slabs of 256^3 on one GPU,
+    against the unsplit 256^3 run; each 30 hipGraph-replayed steps after 3
+    warm-up steps.  The exchange copies stand in for the per-stage neighbour
+    send/recv.
 Each prints one JSON line.
 """
 from __future__ import annotations
@@ -367,9 +374,36 @@ def heat_cases(dropin_steps: int = 5) -> dict:
     return out
 
 
+def slab_cases(steps: int = 30, warmup: int = 3) -> dict:
+    e0, e1 = hip.Event(), hip.Event()
+
+    def rate(solver, npts):
+        solver.set_initial(solvers.ic_fourier((solver.nx, solver.ny, solver.nz), seed=3))
+        solver.steps(warmup)
+        hip.synchronize()
+        e0.record(solver.stream)
+        solver.steps(steps)
+        e1.record(solver.stream)
+        e1.synchronize()
+        return npts * steps / (e0.elapsed_ms(e1) / 1e3) / 1e9
+
+    out = {"study": "slab", "scheme": "WENO5-Z p=2 eps=1e-40", "steps": steps}
+    one = solvers.SlabBurgers((256, 256, 32), 1)
+    out["one_slab_256x256x32_gpts"] = rate(one, 256 * 256 * 32)
+    del one
+    eight = solvers.SlabBurgers((256, 256, 256), 8)
+    out["eight_slabs_256_gpts"] = rate(eight, 256 ** 3)
+    del eight
+    whole = solvers.PeriodicBurgers(256)
+    out["unsplit_256_gpts"] = rate(whole, 256 ** 3)
+    out["one_slab_vs_unsplit"] = out["one_slab_256x256x32_gpts"] / out["unsplit_256_gpts"]
+    out["eight_slabs_vs_unsplit"] = out["eight_slabs_256_gpts"] / out["unsplit_256_gpts"]
+    return out
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
-    ap.add_argument("study", choices=["batched", "fp32", "adjoint", "reference", "heat"])
+    ap.add_argument("study", choices=["batched", "fp32", "adjoint", "reference", "heat", "slab"])
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--n", type=int, default=512)
     args = ap.parse_args(argv)
@@ -379,6 +413,8 @@ def main(argv=None) -> int:
         r = reference_cases()
     elif args.study == "heat":
         r = heat_cases()
+    elif args.study == "slab":
+        r = slab_cases()
     elif args.study == "adjoint":
         r = adjoint(args.n if args.n != 512 else 128)
     else:
