@@ -135,6 +135,56 @@ void Call_RK3d_pitched(dim3 numBlocks, dim3 threadsPerBlock, hipStream_t aStream
                        double *uo, double *Lu);
 
 /* ------------------------------------------------------------------------ */
+/* MultiGPU/Burgers3d_Baseline device wrappers: one rank's z slab            */
+/* ------------------------------------------------------------------------ */
+/* Same names, arguments and update sets as BurgersMPI.h:112-132 /
+ * Kernels.cu:528-580 (launch geometry advisory, errors through
+ * mgad_get_last_error).  A slab holds _Nz planes (3 ghost planes at each end)
+ * of row pitch `pitch`.  The scheme is the one that code is built with:
+ * Reconstruct1d (WENO5-JS, w = alpha/sum then /6), EPS 1e-6, local LF. */
+
+/* Compute_dF / dG (Kernels.cu:225-371): Lu = (x) or Lu -= (y) on lines over
+ * 3..N-4 and planes kstart..kstop-1, nodes 2..N-4 along the line (the face
+ * below node 2 taken as 0). */
+void Compute_Adv_x_Async(dim3 numBlocks, dim3 threadsPerBlock, hipStream_t aStream,
+                         unsigned int pitch, unsigned int Nx, unsigned int Ny,
+                         unsigned int _Nz, unsigned int kstart, unsigned int kstop,
+                         double dx, double *d_s_u, double *d_s_Lu);
+void Compute_Adv_y_Async(dim3 numBlocks, dim3 threadsPerBlock, hipStream_t aStream,
+                         unsigned int pitch, unsigned int Nx, unsigned int Ny,
+                         unsigned int _Nz, unsigned int kstart, unsigned int kstop,
+                         double dy, double *d_s_u, double *d_s_Lu);
+/* Compute_dH (Kernels.cu:376-452): Lu -= on columns 3..N-4 in x and y and
+ * planes kstart..kstop-1 (kstart >= 3, kstop <= _Nz-3), both faces of every
+ * plane computed. */
+void Compute_Adv_z_Async(dim3 numBlocks, dim3 threadsPerBlock, hipStream_t aStream,
+                         unsigned int pitch, unsigned int Nx, unsigned int Ny,
+                         unsigned int _Nz, unsigned int kstart, unsigned int kstop,
+                         double dz, double *d_s_u, double *d_s_Lu);
+/* Compute_RK (Kernels.cu:457-490): the SSP-RK3 stage on every node of the
+ * slab, ghosts included. */
+void Compute_sspRK3Async(dim3 numBlocks, dim3 threadsPerBlock, hipStream_t aStream,
+                         unsigned int step, unsigned int pitch, unsigned int Nx,
+                         unsigned int Ny, unsigned int _Nz, double dt, double *q,
+                         double *qo, double *Lq);
+/* copy_br_to_gc (Kernels.cu:40-72): planes _Nz-6.._Nz-4 (side 0) or 3..5
+ * (side 1) into a 3-plane buffer of row pitch gc_pitch. */
+void CopyBoundaryRegionToGhostCellAsync(dim3 numBlocks, dim3 threadsPerBlock,
+                                        hipStream_t aStream, double *d_s_q,
+                                        double *d_send_buffer, unsigned int Nx,
+                                        unsigned int Ny, unsigned int _Nz,
+                                        unsigned int pitch, unsigned int gc_pitch,
+                                        unsigned int side);
+/* copy_gc_to_br (Kernels.cu:77-107): a 3-plane buffer into planes
+ * _Nz-3.._Nz-1 (side 0) or 0..2 (side 1). */
+void CopyGhostCellToBoundaryRegionAsync(dim3 numBlocks, dim3 threadsPerBlock,
+                                        hipStream_t aStream, double *d_s_q,
+                                        double *d_recv_buffer, unsigned int Nx,
+                                        unsigned int Ny, unsigned int _Nz,
+                                        unsigned int pitch, unsigned int gc_pitch,
+                                        unsigned int side);
+
+/* ------------------------------------------------------------------------ */
 /* Scheme selection for the Call_Adv_* sweeps                                */
 /* ------------------------------------------------------------------------ */
 #define MGAD_WENO_Z     0  /* SingleGPU/Burgers3d_WENO5/kernels.cu:142-192 */

@@ -32,6 +32,8 @@ EXPORTS = (
     "mgad_burgers_ref_steps", "mgad_heat7_fe_steps", "mgad_heat13_rk3_steps", "mgad_check_division", "mgad_ic_fourier", "mgad_ic_gauss_noise",
     "mgad_save_binary3d", "mgad_load_binary3d", "mgad_init3d_burgers", "mgad_init3d_heat", "mgad_heat_error_norms",
     "mgad_slab_exchange", "mgad_burgers_slab_init", "mgad_burgers_slab_steps",
+    "Compute_Adv_x_Async", "Compute_Adv_y_Async", "Compute_Adv_z_Async", "Compute_sspRK3Async",
+    "CopyBoundaryRegionToGhostCellAsync", "CopyGhostCellToBoundaryRegionAsync",
 )
 
 
@@ -110,6 +112,16 @@ def lib():
         L.Call_Lu3d_pitched.restype = None
         L.Call_RK3d_pitched.argtypes = [Dim3, Dim3, VP, U, U, U, U, U, D, VP, VP, VP]
         L.Call_RK3d_pitched.restype = None
+        for name in ("Compute_Adv_x_Async", "Compute_Adv_y_Async", "Compute_Adv_z_Async"):
+            f = getattr(L, name)
+            f.argtypes = [Dim3, Dim3, VP, U, U, U, U, U, U, D, VP, VP]
+            f.restype = None
+        L.Compute_sspRK3Async.argtypes = [Dim3, Dim3, VP, U, U, U, U, U, D, VP, VP, VP]
+        L.Compute_sspRK3Async.restype = None
+        for name in ("CopyBoundaryRegionToGhostCellAsync", "CopyGhostCellToBoundaryRegionAsync"):
+            f = getattr(L, name)
+            f.argtypes = [Dim3, Dim3, VP, VP, VP, U, U, U, U, U, U]
+            f.restype = None
         L.mgad_set_adv_scheme.argtypes = [I, I, D, I, D]
         L.mgad_set_adv_scheme.restype = I
         L.mgad_lu3d_o2.argtypes = [VP, U, U, U, VP, VP]

@@ -33,25 +33,31 @@ constexpr int kChunk = 32;  // faces per thread and line chunk
 
 }  // namespace
 
+// Which nodes a sweep updates: lines over the first transverse coordinate
+// a = 3..n_a-4 and the second b = b0..b0+nb-1; along each line nodes l0..l1-1,
+// using faces l0-1/2 .. l1-1/2.  zero_first: the face below node l0 is 0 (the
+// reference's fu_old = 0, SingleGPU/Burgers3d_WENO5/kernels.cu:222) instead
+// of being computed from nodes l0-3..l0+2.
+struct SweepRange {
+    int b0, nb, l0, l1, zero_first;
+};
+
 // AXIS 0: x lines (rows), 1: y lines, 2: z lines.  MODE 0 writes Lu, 1 subtracts.
 // px is the row pitch in elements (px >= nx; px = nx for unpitched arrays).
 template <int AXIS, int WENO, int P, bool LOCAL>
 __global__ __launch_bounds__(256) void k_adv_sweep(const double *__restrict__ u,
                                                    double *__restrict__ Lu, int nx, int ny, int nz,
                                                    int px, double d, double eps, double alpha,
-                                                   int nchunks) {
-    // transverse extents (interior 3..n-4) and line length
-    const int n = AXIS == 0 ? nx : (AXIS == 1 ? ny : nz);
-    const int na = (AXIS == 0 ? ny : nx) - 6;        // first transverse coordinate count
-    const int nb = (AXIS == 2 ? ny : nz) - 6;        // second transverse coordinate count
+                                                   SweepRange r, int nchunks) {
+    const int na = (AXIS == 0 ? ny : nx) - 6;  // first transverse coordinate count (3..n-4)
     const long gid = static_cast<long>(blockIdx.x) * 256 + threadIdx.x;
-    const long nlines = static_cast<long>(na) * nb;
+    const long nlines = static_cast<long>(na) * r.nb;
     if (gid >= nlines * nchunks) return;
     // lanes run fastest over the first transverse coordinate (x for y/z sweeps)
     const int a = 3 + static_cast<int>(gid % na);
     const long rest = gid / na;
-    const int b = 3 + static_cast<int>(rest % nb);
-    const int c = static_cast<int>(rest / nb);
+    const int b = r.b0 + static_cast<int>(rest % r.nb);
+    const int c = static_cast<int>(rest / r.nb);
 
     const unsigned plane = static_cast<unsigned>(px) * ny;
     unsigned base, st;
@@ -65,10 +71,10 @@ __global__ __launch_bounds__(256) void k_adv_sweep(const double *__restrict__ u,
         base = static_cast<unsigned>(a) + static_cast<unsigned>(px) * b;
         st = plane;
     }
-    // faces i+1/2 for i in [i0, i1): the reference marches i = 2..n-4
-    const int i0 = 2 + c * kChunk;
-    const int i1 = min(i0 + kChunk, n - 3);
+    const int i0 = r.l0 + c * kChunk;
+    const int i1 = min(i0 + kChunk, r.l1);
     if (i0 >= i1) return;
+    const bool low_face = i0 > r.l0 || !r.zero_first;
 
     auto split = [&](double v, double &fp, double &fm) {
         if (LOCAL)
@@ -76,16 +82,15 @@ __global__ __launch_bounds__(256) void k_adv_sweep(const double *__restrict__ u,
         else
             lf_split(v, alpha, fp, fm);
     };
-    // queue: fp[q], fm[q] hold node i-3+q for the face being computed
-    // (node i0-3 exists only for chunks after the first; the first chunk's
-    // low face is the reference's fu_old = 0 and never reads it)
+    // queue: fp[q], fm[q] hold node i-3+q for the face being computed (node
+    // i0-3 is read only when the chunk's low face is computed)
     double fp[7], fm[7];
     fp[0] = fm[0] = 0.0;
-    if (i0 > 2) split(u[base + st * static_cast<unsigned>(i0 - 3)], fp[0], fm[0]);
+    if (low_face) split(u[base + st * static_cast<unsigned>(i0 - 3)], fp[0], fm[0]);
 #pragma unroll
     for (int q = 1; q < 6; q++) split(u[base + st * static_cast<unsigned>(i0 - 3 + q)], fp[q], fm[q]);
     double fu_old = 0.0;
-    if (i0 > 2)
+    if (low_face)
         fu_old = weno_face<WENO, P>(fp[0], fp[1], fp[2], fp[3], fp[4], fm[1], fm[2], fm[3], fm[4],
                                     fm[5], eps);
     for (int i = i0; i < i1; i++) {
@@ -174,19 +179,17 @@ namespace {
 
 template <int AXIS, int WENO, int P>
 void launch_sweep_t(hipStream_t s, unsigned nx, unsigned ny, unsigned nz, unsigned px, double d,
-                    double *u, double *Lu) {
-    const int n = AXIS == 0 ? nx : (AXIS == 1 ? ny : nz);
-    const long na = (AXIS == 0 ? ny : nx) - 6, nb = (AXIS == 2 ? ny : nz) - 6;
-    const int nchunks = (n - 5 + kChunk - 1) / kChunk;  // faces 2..n-4
-    const long total = na * nb * nchunks;
+                    double *u, double *Lu, const AdvScheme &sc, SweepRange r) {
+    const long na = (AXIS == 0 ? ny : nx) - 6;
+    const int nchunks = (r.l1 - r.l0 + kChunk - 1) / kChunk;
+    const long total = na * r.nb * nchunks;
     const unsigned blocks = static_cast<unsigned>((total + 255) / 256);
-    if (g_adv.lf == MGAD_LF_LOCAL)
+    if (sc.lf == MGAD_LF_LOCAL)
         hipLaunchKernelGGL((k_adv_sweep<AXIS, WENO, P, true>), dim3(blocks), dim3(256), 0, s, u, Lu,
-                           (int)nx, (int)ny, (int)nz, (int)px, d, g_adv.eps, g_adv.alpha, nchunks);
+                           (int)nx, (int)ny, (int)nz, (int)px, d, sc.eps, sc.alpha, r, nchunks);
     else
         hipLaunchKernelGGL((k_adv_sweep<AXIS, WENO, P, false>), dim3(blocks), dim3(256), 0, s, u,
-                           Lu, (int)nx, (int)ny, (int)nz, (int)px, d, g_adv.eps, g_adv.alpha,
-                           nchunks);
+                           Lu, (int)nx, (int)ny, (int)nz, (int)px, d, sc.eps, sc.alpha, r, nchunks);
 }
 
 // Arrays of px*ny*nz elements (row pitch px >= nx) addressable with 32-bit offsets.
@@ -201,12 +204,16 @@ void launch_sweep(hipStream_t s, unsigned nx, unsigned ny, unsigned nz, unsigned
         set_error(MGAD_ERR_INVALID_ARG);
         return;
     }
+    // the reference's sweeps: lines over 3..n-4 both ways, nodes 2..n-4 along
+    // the line with the face below node 2 taken as 0
+    const int n = AXIS == 0 ? nx : (AXIS == 1 ? ny : nz);
+    const SweepRange r{3, static_cast<int>(AXIS == 2 ? ny : nz) - 6, 2, n - 3, 1};
     if (g_adv.weno == MGAD_WENO_JS)
-        launch_sweep_t<AXIS, kWenoJS, 2>(s, nx, ny, nz, px, d, u, Lu);
+        launch_sweep_t<AXIS, kWenoJS, 2>(s, nx, ny, nz, px, d, u, Lu, g_adv, r);
     else if (g_adv.p == 2)
-        launch_sweep_t<AXIS, kWenoZ, 2>(s, nx, ny, nz, px, d, u, Lu);
+        launch_sweep_t<AXIS, kWenoZ, 2>(s, nx, ny, nz, px, d, u, Lu, g_adv, r);
     else
-        launch_sweep_t<AXIS, kWenoZ, 1>(s, nx, ny, nz, px, d, u, Lu);
+        launch_sweep_t<AXIS, kWenoZ, 1>(s, nx, ny, nz, px, d, u, Lu, g_adv, r);
     check_launch();
 }
 
@@ -255,6 +262,82 @@ void rk_heat(hipStream_t s, unsigned step, unsigned nx, unsigned ny, unsigned nz
     const dim3 grid((nx - 4 + 63) / 64, (ny - 4 + 3) / 4, nz - 4);
     hipLaunchKernelGGL(k_rk_interior, grid, dim3(256), 0, s, u, uo, Lu, (int)nx, (int)ny, (int)nz,
                        (int)px, 2, (int)step, dt);
+    check_launch();
+}
+
+// ---- MultiGPU/Burgers3d_Baseline: one rank's z slab of _Nz planes (3 ghost
+// planes at each end), row pitch `pitch`; its scheme is fixed at compile time
+// there: Reconstruct1d (WENO5-JS, Kernels.cu:112-220), EPS 1e-6
+// (BurgersMPI.h:27), local Lax-Friedrichs split (Kernels.cu:254-262). ----
+
+const AdvScheme kMb3Scheme{kWenoJS, 2, 1e-6, MGAD_LF_LOCAL, 0.0};
+
+// Compute_dF/dG (Kernels.cu:225-371): lines over 3..n-4 and planes
+// kstart..kstop-1, nodes 2..n-4 along the line from fu_old = 0.
+// Compute_dH (:376-452): columns 3..n-4 in x and y, nodes kstart..kstop-1,
+// the face below kstart computed from planes kstart-3..kstart+2.
+template <int AXIS>
+void mb3_sweep(hipStream_t s, unsigned pitch, unsigned nx, unsigned ny, unsigned nz, unsigned kstart,
+               unsigned kstop, double d, double *u, double *Lu) {
+    const bool range_ok = AXIS == 2 ? (kstart >= 3 && kstop + 3 <= nz) : kstop <= nz;
+    if (!grid_ok(nx, ny, nz, pitch, 7) || !u || !Lu || kstart >= kstop || !range_ok) {
+        set_error(MGAD_ERR_INVALID_ARG);
+        return;
+    }
+    SweepRange r;
+    if (AXIS == 2)
+        r = SweepRange{3, static_cast<int>(ny) - 6, static_cast<int>(kstart), static_cast<int>(kstop), 0};
+    else
+        r = SweepRange{static_cast<int>(kstart), static_cast<int>(kstop - kstart), 2,
+                       static_cast<int>(AXIS == 0 ? nx : ny) - 3, 1};
+    launch_sweep_t<AXIS, kWenoJS, 2>(s, nx, ny, nz, pitch, d, u, Lu, kMb3Scheme, r);
+    check_launch();
+}
+
+// Compute_RK (Kernels.cu:457-490): every node of the slab, ghosts included.
+__global__ __launch_bounds__(256) void k_rk_all(double *q, const double *qo, const double *__restrict__ Lq, int nx,
+                                                int ny, int nz, int px, int step, double dt) {
+    const int i = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int k = blockIdx.z;
+    if (i >= nx || j >= ny || k >= nz) return;
+    const unsigned o = i + static_cast<unsigned>(px) * j + static_cast<unsigned>(px) * ny * k;
+    q[o] = rk_stage(step, dt, q[o], qo[o], Lq[o]);
+}
+
+// copy_br_to_gc / copy_gc_to_br (Kernels.cu:40-107): the 3 boundary planes
+// of side 0 (top: planes _Nz-6.._Nz-4 out, ghosts _Nz-3.._Nz-1 in) or side 1
+// (bottom: planes 3..5 out, ghosts 0..2 in) to or from a 3-plane buffer of
+// row pitch gc_pitch.
+template <bool TO_BUFFER>
+__global__ __launch_bounds__(256) void k_halo_copy(double *q, double *buf, int nx, int ny, int nz, int px,
+                                                   int gpx, int side) {
+    const int i = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (i >= nx || j >= ny) return;
+    const int k0 = TO_BUFFER ? (side == 0 ? nz - 6 : 3) : (side == 0 ? nz - 3 : 0);
+    const unsigned xy = static_cast<unsigned>(px) * ny, gxy = static_cast<unsigned>(gpx) * ny;
+#pragma unroll
+    for (int g = 0; g < 3; g++) {
+        const unsigned o = i + static_cast<unsigned>(px) * j + xy * static_cast<unsigned>(k0 + g);
+        const unsigned b = i + static_cast<unsigned>(gpx) * j + gxy * static_cast<unsigned>(g);
+        if (TO_BUFFER)
+            buf[b] = q[o];
+        else
+            q[o] = buf[b];
+    }
+}
+
+template <bool TO_BUFFER>
+void halo_copy(hipStream_t s, double *q, double *buf, unsigned nx, unsigned ny, unsigned nz, unsigned pitch,
+               unsigned gc_pitch, unsigned side) {
+    if (!grid_ok(nx, ny, nz, pitch, 7) || gc_pitch < nx || 3ull * gc_pitch * ny >= (1ull << 31) || !q || !buf ||
+        side > 1) {
+        set_error(MGAD_ERR_INVALID_ARG);
+        return;
+    }
+    hipLaunchKernelGGL(k_halo_copy<TO_BUFFER>, dim3((nx + 63) / 64, (ny + 3) / 4), dim3(256), 0, s, q, buf,
+                       (int)nx, (int)ny, (int)nz, (int)pitch, (int)gc_pitch, (int)side);
     check_launch();
 }
 
@@ -368,4 +451,50 @@ extern "C" int mgad_lu3d_o2(hipStream_t stream, unsigned int nx, unsigned int ny
     hipLaunchKernelGGL(k_lu3d_o2, grid, dim3(256), 0, stream, u, Lu, (int)nx, (int)ny, (int)nz,
                        (int)nx, c.kx, c.ky, c.kz);
     return check_launch();
+}
+
+// ---- MultiGPU/Burgers3d_Baseline (BurgersMPI.h:112-132): one rank's slab ----
+
+extern "C" void Compute_Adv_x_Async(dim3, dim3, hipStream_t aStream, unsigned int pitch, unsigned int Nx,
+                                    unsigned int Ny, unsigned int _Nz, unsigned int kstart, unsigned int kstop,
+                                    double dx, double *d_s_u, double *d_s_Lu) {
+    mb3_sweep<0>(aStream, pitch, Nx, Ny, _Nz, kstart, kstop, dx, d_s_u, d_s_Lu);
+}
+
+extern "C" void Compute_Adv_y_Async(dim3, dim3, hipStream_t aStream, unsigned int pitch, unsigned int Nx,
+                                    unsigned int Ny, unsigned int _Nz, unsigned int kstart, unsigned int kstop,
+                                    double dy, double *d_s_u, double *d_s_Lu) {
+    mb3_sweep<1>(aStream, pitch, Nx, Ny, _Nz, kstart, kstop, dy, d_s_u, d_s_Lu);
+}
+
+extern "C" void Compute_Adv_z_Async(dim3, dim3, hipStream_t aStream, unsigned int pitch, unsigned int Nx,
+                                    unsigned int Ny, unsigned int _Nz, unsigned int kstart, unsigned int kstop,
+                                    double dz, double *d_s_u, double *d_s_Lu) {
+    mb3_sweep<2>(aStream, pitch, Nx, Ny, _Nz, kstart, kstop, dz, d_s_u, d_s_Lu);
+}
+
+extern "C" void Compute_sspRK3Async(dim3, dim3, hipStream_t aStream, unsigned int step, unsigned int pitch,
+                                    unsigned int Nx, unsigned int Ny, unsigned int _Nz, double dt, double *q,
+                                    double *qo, double *Lq) {
+    if (!grid_ok(Nx, Ny, _Nz, pitch, 1) || step < 1 || step > 3 || !q || !qo || !Lq || _Nz > 65535) {
+        set_error(MGAD_ERR_INVALID_ARG);
+        return;
+    }
+    hipLaunchKernelGGL(k_rk_all, dim3((Nx + 63) / 64, (Ny + 3) / 4, _Nz), dim3(256), 0, aStream, q, qo, Lq, (int)Nx,
+                       (int)Ny, (int)_Nz, (int)pitch, (int)step, dt);
+    check_launch();
+}
+
+extern "C" void CopyBoundaryRegionToGhostCellAsync(dim3, dim3, hipStream_t aStream, double *d_s_q,
+                                                   double *d_send_buffer, unsigned int Nx, unsigned int Ny,
+                                                   unsigned int _Nz, unsigned int pitch, unsigned int gc_pitch,
+                                                   unsigned int side) {
+    halo_copy<true>(aStream, d_s_q, d_send_buffer, Nx, Ny, _Nz, pitch, gc_pitch, side);
+}
+
+extern "C" void CopyGhostCellToBoundaryRegionAsync(dim3, dim3, hipStream_t aStream, double *d_s_q,
+                                                   double *d_recv_buffer, unsigned int Nx, unsigned int Ny,
+                                                   unsigned int _Nz, unsigned int pitch, unsigned int gc_pitch,
+                                                   unsigned int side) {
+    halo_copy<false>(aStream, d_s_q, d_recv_buffer, Nx, Ny, _Nz, pitch, gc_pitch, side);
 }

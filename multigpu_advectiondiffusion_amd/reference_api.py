@@ -160,3 +160,37 @@ def heat_reference_step(u, uo, Lu, nx, ny, nz, dt, stream=None) -> None:
     for step in (1, 2, 3):
         Call_Lu3d(None, None, stream, nx, ny, nz, u, Lu)
         Call_RK3d(None, None, stream, step, nx, ny, nz, dt, u, uo, Lu)
+
+
+# ---- MultiGPU/Burgers3d_Baseline device wrappers (BurgersMPI.h:112-132) ----
+
+def Compute_Adv_x_Async(numBlocks, threadsPerBlock, aStream, pitch, Nx, Ny, _Nz, kstart, kstop, dx, u, Lu) -> None:
+    _call("Compute_Adv_x_Async", _dim(numBlocks), _dim(threadsPerBlock), _stream(aStream), pitch, Nx, Ny, _Nz,
+          kstart, kstop, dx, _ptr(u), _ptr(Lu))
+
+
+def Compute_Adv_y_Async(numBlocks, threadsPerBlock, aStream, pitch, Nx, Ny, _Nz, kstart, kstop, dy, u, Lu) -> None:
+    _call("Compute_Adv_y_Async", _dim(numBlocks), _dim(threadsPerBlock), _stream(aStream), pitch, Nx, Ny, _Nz,
+          kstart, kstop, dy, _ptr(u), _ptr(Lu))
+
+
+def Compute_Adv_z_Async(numBlocks, threadsPerBlock, aStream, pitch, Nx, Ny, _Nz, kstart, kstop, dz, u, Lu) -> None:
+    _call("Compute_Adv_z_Async", _dim(numBlocks), _dim(threadsPerBlock), _stream(aStream), pitch, Nx, Ny, _Nz,
+          kstart, kstop, dz, _ptr(u), _ptr(Lu))
+
+
+def Compute_sspRK3Async(numBlocks, threadsPerBlock, aStream, step, pitch, Nx, Ny, _Nz, dt, q, qo, Lq) -> None:
+    _call("Compute_sspRK3Async", _dim(numBlocks), _dim(threadsPerBlock), _stream(aStream), step, pitch, Nx, Ny, _Nz,
+          dt, _ptr(q), _ptr(qo), _ptr(Lq))
+
+
+def CopyBoundaryRegionToGhostCellAsync(numBlocks, threadsPerBlock, aStream, d_s_q, d_send_buffer, Nx, Ny, _Nz,
+                                       pitch, gc_pitch, side) -> None:
+    _call("CopyBoundaryRegionToGhostCellAsync", _dim(numBlocks), _dim(threadsPerBlock), _stream(aStream),
+          _ptr(d_s_q), _ptr(d_send_buffer), Nx, Ny, _Nz, pitch, gc_pitch, side)
+
+
+def CopyGhostCellToBoundaryRegionAsync(numBlocks, threadsPerBlock, aStream, d_s_q, d_recv_buffer, Nx, Ny, _Nz,
+                                       pitch, gc_pitch, side) -> None:
+    _call("CopyGhostCellToBoundaryRegionAsync", _dim(numBlocks), _dim(threadsPerBlock), _stream(aStream),
+          _ptr(d_s_q), _ptr(d_recv_buffer), Nx, Ny, _Nz, pitch, gc_pitch, side)

@@ -80,6 +80,8 @@ def lib():
         L.orc_lu3d_13.argtypes = [_U, _U, _U, _D, _D, _D, _P, _P]
         L.orc_lu3d_o2.argtypes = [_U, _U, _U, _D, _D, _D, _P, _P]
         L.orc_rk3d.argtypes = [_U, _U, _U, _U, _D, _P, _P, _P]
+        L.orc_mb3_adv.argtypes = [ctypes.c_int, _U, _U, _U, _U, _U, _U, _D, _P, _P, _S]
+        L.orc_mb3_rk.argtypes = [_U, _U, _U, _U, _U, _D, _P, _P, _P]
         L.orc_maxabs.restype = _D
         L.orc_maxabs.argtypes = [ctypes.c_uint64, _P]
         L.orc_rhs_periodic.argtypes = [_U, _U, _U, _D, _D, _D, _P, _P, _S]
@@ -150,6 +152,26 @@ def adv(axis: int, u: np.ndarray, Lu: np.ndarray, d: float, **kw) -> None:
     s = scheme(**kw)
     fn = (lib().orc_adv_x, lib().orc_adv_y, lib().orc_adv_z)[axis]
     fn(*_shape(u), d, _p(u), _p(Lu), ctypes.byref(s))
+
+
+# MultiGPU/Burgers3d_Baseline (BurgersMPI.h:21-29, Kernels.cu:112-262):
+# Reconstruct1d (WENO5-JS), EPS 1e-6, local LF.
+MB3 = dict(weno=WENO_JS, p=2, eps=1e-6, lf=LF_LOCAL)
+
+
+def mb3_adv(axis: int, u: np.ndarray, Lu: np.ndarray, kstart: int, kstop: int, d: float, pitch: int | None = None,
+            nx: int | None = None) -> None:
+    """Compute_Adv_{x,y,z}_Async of one rank's slab (in place on Lu); u and Lu
+    are (nz, ny, pitch) arrays of which the first nx columns are the grid."""
+    nz, ny, px = u.shape
+    s = scheme(**MB3)
+    lib().orc_mb3_adv(axis, px if pitch is None else pitch, px if nx is None else nx, ny, nz, kstart, kstop, d,
+                      _p(u), _p(Lu), ctypes.byref(s))
+
+
+def mb3_rk(q, qo, Lq, step, dt, nx: int | None = None) -> None:
+    nz, ny, px = q.shape
+    lib().orc_mb3_rk(step, px, px if nx is None else nx, ny, nz, dt, _p(q), _p(qo), _p(Lq))
 
 
 def diff13_b5(u, Lu, kx, ky, kz) -> None:

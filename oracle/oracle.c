@@ -24,6 +24,9 @@
  *                                       telescoping of the written differences.
  *  - orc_rhs_periodic ............... sum(Lu) = 0 (periodic conservation); permutation
  *                                       invariance; 1-D data reduces to orc_rhs_1d.
+ *  - orc_mb3_adv, orc_mb3_rk ........ x/y on planes 3..nz-4 with pitch nx equal orc_adv_x/y;
+ *                                       z equals orc_adv_z on planes 3..nz-4 and splits over
+ *                                       plane ranges; RK equals rk_node on every node.
  *  - orc_ssprk_b5, orc_rk3d,
  *    orc_rk_stage_all ............... linear ODE u' = lambda*u gives the SSP-RK3
  *                                       amplification 1 + z + z^2/2 + z^3/6.
@@ -342,6 +345,67 @@ void orc_rk3d(unsigned step, unsigned nx, unsigned ny, unsigned nz, double dt,
             for (unsigned i = 2; i < nx - 2; i++) {
                 size_t o = i + (size_t)nx * j + (size_t)nx * ny * k;
                 u[o] = rk_node(step, dt, u[o], uo[o], Lu[o]);
+            }
+}
+
+/* ------------------------------------------------------------------------ */
+/* MultiGPU/Burgers3d_Baseline: one rank's z slab (pitched)                 */
+/* ------------------------------------------------------------------------ */
+
+/* Compute_dH's line form, MB3/Kernels.cu:376-452: nodes k0..k1-1 of a line,
+ * the face below k0 computed from nodes k0-3..k0+2 (not taken as 0). */
+static void march_line_from(const double *u, double *Lu, size_t base, size_t st, unsigned k0,
+                            unsigned k1, double d, const orc_scheme *s) {
+    double fp[5], fm[5], dummy;
+    for (int q = 0; q < 5; q++) orc_split(u[base + st * (k0 - 3 + q)], s->alpha, s->lf, &fp[q], &dummy);
+    for (int q = 0; q < 5; q++) orc_split(u[base + st * (k0 - 2 + q)], s->alpha, s->lf, &dummy, &fm[q]);
+    double fu_old = orc_face(fp, fm, s);
+    for (unsigned k = k0; k < k1; k++) {
+        for (int q = 0; q < 4; q++) { fp[q] = fp[q + 1]; fm[q] = fm[q + 1]; }
+        orc_split(u[base + st * (k + 2)], s->alpha, s->lf, &fp[4], &dummy);
+        orc_split(u[base + st * (k + 3)], s->alpha, s->lf, &dummy, &fm[4]);
+        double fu = orc_face(fp, fm, s);
+        Lu[base + st * k] -= (fu - fu_old) / d;
+        fu_old = fu;
+    }
+}
+
+/* Compute_dF (MB3/Kernels.cu:225-293, axis 0: rows 3<=j<ny-3, Lu =),
+ * Compute_dG (:298-371, axis 1: columns 3<=i<nx-3, Lu -=), both on planes
+ * kstart..kstop-1 with i (j) = 2..n-4 from fu_old = 0; Compute_dH (:376-452,
+ * axis 2: columns 3<=i<nx-3, 3<=j<ny-3, planes kstart..kstop-1).  pitch is the
+ * row pitch of u and Lu. */
+void orc_mb3_adv(int axis, unsigned pitch, unsigned nx, unsigned ny, unsigned nz, unsigned kstart,
+                 unsigned kstop, double d, const double *u, double *Lu, const orc_scheme *s) {
+    if (nx < 7 || ny < 7 || nz < 7 || kstart >= kstop) return;
+    const size_t XY = (size_t)pitch * ny;
+    if (axis == 0) {
+#pragma omp parallel for collapse(2) schedule(static)
+        for (unsigned k = kstart; k < kstop; k++)
+            for (unsigned j = 3; j < ny - 3; j++)
+                march_line(u, Lu, (size_t)pitch * j + XY * k, 1, nx, d, 0, s);
+    } else if (axis == 1) {
+#pragma omp parallel for collapse(2) schedule(static)
+        for (unsigned k = kstart; k < kstop; k++)
+            for (unsigned i = 3; i < nx - 3; i++)
+                march_line(u, Lu, i + XY * k, pitch, ny, d, 1, s);
+    } else {
+#pragma omp parallel for collapse(2) schedule(static)
+        for (unsigned j = 3; j < ny - 3; j++)
+            for (unsigned i = 3; i < nx - 3; i++)
+                march_line_from(u, Lu, i + (size_t)pitch * j, XY, kstart, kstop, d, s);
+    }
+}
+
+/* Compute_RK, MB3/Kernels.cu:457-490: every node of the slab, ghosts included. */
+void orc_mb3_rk(unsigned step, unsigned pitch, unsigned nx, unsigned ny, unsigned nz, double dt,
+                double *q, const double *qo, const double *Lq) {
+#pragma omp parallel for collapse(2) schedule(static)
+    for (unsigned k = 0; k < nz; k++)
+        for (unsigned j = 0; j < ny; j++)
+            for (unsigned i = 0; i < nx; i++) {
+                size_t o = i + (size_t)pitch * j + (size_t)pitch * ny * k;
+                q[o] = rk_node(step, dt, q[o], qo[o], Lq[o]);
             }
 }
 

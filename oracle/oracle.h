@@ -75,6 +75,14 @@ void orc_lu3d_o2(unsigned nx, unsigned ny, unsigned nz, double kx, double ky,
 void orc_rk3d(unsigned step, unsigned nx, unsigned ny, unsigned nz, double dt,
               double *u, const double *uo, const double *Lu);
 
+/* ---- MultiGPU/Burgers3d_Baseline: one rank's slab, row pitch `pitch` ---- */
+void orc_mb3_adv(int axis, unsigned pitch, unsigned nx, unsigned ny, unsigned nz,
+                 unsigned kstart, unsigned kstop, double d, const double *u,
+                 double *Lu, const orc_scheme *s);
+void orc_mb3_rk(unsigned step, unsigned pitch, unsigned nx, unsigned ny,
+                unsigned nz, double dt, double *q, const double *qo,
+                const double *Lq);
+
 /* ---- periodic global-LF Burgers (the fused path's specification) ---- */
 double orc_maxabs(uint64_t n, const double *u);
 void   orc_rhs_periodic(unsigned nx, unsigned ny, unsigned nz, double dx,

@@ -191,3 +191,73 @@ def test_heat7_max_principle():
     k = 1.0 / (12 * dx * dx)
     O.heat7_fe(u, k, k, k, 0.15 * dx * dx, 50)
     assert np.abs(u).max() <= m0
+
+
+# ------------------------------------- MultiGPU/Burgers3d_Baseline slab sweeps
+def test_mb3_xy_equal_single_grid_sweeps():
+    """Compute_dF/dG over planes 3..nz-4 at pitch nx are the single-GPU sweeps
+    with the same scheme (MB3/Kernels.cu:225-371 vs B5/kernels.cu:197-327)."""
+    nx, ny, nz = 13, 11, 12
+    u = rnd((nx, ny, nz), 21)
+    for axis in (0, 1):
+        L0 = rnd((nx, ny, nz), 22)
+        L1 = L0.copy()
+        O.adv(axis, u, L0, 0.07, **O.MB3)
+        O.mb3_adv(axis, u, L1, 3, nz - 3, 0.07)
+        assert np.array_equal(L0, L1)
+
+
+def test_mb3_pitch_padding_never_read():
+    """Rows padded to pitch nx+5 with NaN: the same values on the grid, the
+    padding of Lu untouched, for all three sweeps and the RK stage."""
+    nx, ny, nz, px = 12, 10, 14, 17
+    u = rnd((nx, ny, nz), 23)
+    up = np.full((nz, ny, px), np.nan)
+    up[:, :, :nx] = u
+    for axis in (0, 1, 2):
+        L = rnd((nx, ny, nz), 24)
+        Lp = np.full((nz, ny, px), -5.0)
+        Lp[:, :, :nx] = L
+        O.mb3_adv(axis, u, L, 3, nz - 3, 0.1)
+        O.mb3_adv(axis, up, Lp, 3, nz - 3, 0.1, nx=nx)
+        assert np.array_equal(Lp[:, :, :nx], L) and np.all(Lp[:, :, nx:] == -5.0)
+    qo, Lq = rnd((nx, ny, nz), 25), rnd((nx, ny, nz), 26)
+    q = u.copy()
+    O.mb3_rk(q, qo, Lq, 2, 0.01)
+    qp = np.full((nz, ny, px), 3.0)
+    qp[:, :, :nx] = u
+    qop, Lqp = qp.copy(), qp.copy()
+    qop[:, :, :nx], Lqp[:, :, :nx] = qo, Lq
+    O.mb3_rk(qp, qop, Lqp, 2, 0.01, nx=nx)
+    assert np.array_equal(qp[:, :, :nx], q) and np.all(qp[:, :, nx:] == 3.0)
+
+
+def test_mb3_z_equals_single_grid_sweep_and_splits():
+    """Compute_dH over planes 3..nz-4 gives the single-GPU z sweep there
+    (both faces of each plane; only the single-GPU sweep also writes plane 2
+    from F = 0), and two adjacent plane ranges give the whole range."""
+    nx, ny, nz = 11, 12, 16
+    u = rnd((nx, ny, nz), 27)
+    L0 = rnd((nx, ny, nz), 28)
+    L1, L2 = L0.copy(), L0.copy()
+    Lb = L0.copy()
+    O.adv(2, u, Lb, 0.09, **O.MB3)
+    O.mb3_adv(2, u, L1, 3, nz - 3, 0.09)
+    assert np.array_equal(L1[3:nz - 3], Lb[3:nz - 3])
+    assert np.array_equal(L1[:3], L0[:3]) and np.array_equal(L1[nz - 3:], L0[nz - 3:])
+    O.mb3_adv(2, u, L2, 3, 7, 0.09)
+    O.mb3_adv(2, u, L2, 7, nz - 3, 0.09)
+    assert np.array_equal(L1, L2)
+
+
+def test_mb3_rk_every_node():
+    """Compute_RK of the MB3 slab (Kernels.cu:457-490) is the single-GPU stage
+    on the interior and advances the ghost nodes too."""
+    n = 10
+    q0, qo, Lq = rnd((n, n, n), 29), rnd((n, n, n), 30), rnd((n, n, n), 31)
+    for step in (1, 2, 3):
+        a, b = q0.copy(), q0.copy()
+        O.ssprk_b5(a, qo, Lq, step, 0.02)
+        O.mb3_rk(b, qo, Lq, step, 0.02)
+        assert np.array_equal(a[3:n - 3, 3:n - 3, 3:n - 3], b[3:n - 3, 3:n - 3, 3:n - 3])
+        assert np.all(b[0] != q0[0])
