@@ -385,6 +385,39 @@ int mgad_burgers_slab_steps(hipStream_t stream, const mgad_scheme *scheme,
                             int use_graph);
 
 /* ------------------------------------------------------------------------ */
+/* WENO7 Burgers (Matlab_Prototipes/InviscidBurgersNd/LFWENO7FDM3d.m)        */
+/* ------------------------------------------------------------------------ */
+/* u_t + (u^2/2)_x + (u^2/2)_y + (u^2/2)_z = 0 on a node grid: local LF split,
+ * WENO7 faces (Balsara-Shu smoothness forms, eps 1e-6, WENO7resAdv_X.m), ends
+ * extended by repeating the boundary value, the face below node 0 of each
+ * line taken as 0, and the low-storage 5-stage 4th-order Runge-Kutta with
+ * dt = cfl*dx/max|u| clipped to tend (LFWENO7FDM3d.m:66-108).  Fields are
+ * nx*ny*nz doubles, x fastest; nx*ny*nz < 2^31. */
+
+/* Starts a run at t0: ctrl (MGAD_CTRL_BYTES) from max|u|, res zeroed. */
+int mgad_weno7_init(hipStream_t stream, unsigned int nx, unsigned int ny,
+                    unsigned int nz, const double *u, double *res, void *ctrl,
+                    double t0, double tend, double cfl);
+
+/* The right-hand side L = dF + dG + dH of one field (res = L), as the
+ * prototype's WENO7resAdv_X + _Y + _Z. */
+int mgad_weno7_rhs(hipStream_t stream, unsigned int nx, unsigned int ny,
+                   unsigned int nz, double dx, double dy, double dz,
+                   const double *u, double *L);
+
+/* nsteps steps, five stage launches each: u (in/out), scratch fields w1, w2
+ * and the carried residual res; steps after tend are no-ops on the device.
+ * use_graph captures one step in a hipGraph and replays it. */
+int mgad_weno7_steps(hipStream_t stream, unsigned int nx, unsigned int ny,
+                     unsigned int nz, double dx, double dy, double dz,
+                     double *u, double *w1, double *w2, double *res,
+                     void *ctrl, int nsteps, int use_graph);
+
+/* t, dt of the last step and the step count (synchronises the stream). */
+int mgad_weno7_read_ctrl(hipStream_t stream, const void *ctrl, double *t,
+                         double *dt, unsigned long long *steps);
+
+/* ------------------------------------------------------------------------ */
 /* Discrete adjoint of the periodic Burgers run (FP64)                       */
 /* ------------------------------------------------------------------------ */
 

@@ -34,6 +34,7 @@ EXPORTS = (
     "mgad_slab_exchange", "mgad_burgers_slab_init", "mgad_burgers_slab_steps",
     "Compute_Adv_x_Async", "Compute_Adv_y_Async", "Compute_Adv_z_Async", "Compute_sspRK3Async",
     "CopyBoundaryRegionToGhostCellAsync", "CopyGhostCellToBoundaryRegionAsync",
+    "mgad_weno7_init", "mgad_weno7_rhs", "mgad_weno7_steps", "mgad_weno7_read_ctrl",
 )
 
 
@@ -169,6 +170,15 @@ def lib():
         L.mgad_burgers_slab_init.restype = I
         L.mgad_burgers_slab_steps.argtypes = [VP, ctypes.POINTER(Scheme), U, U, U, U, D, D, D, VP, VP, VP, VP, I, I]
         L.mgad_burgers_slab_steps.restype = I
+        L.mgad_weno7_init.argtypes = [VP, U, U, U, VP, VP, VP, D, D, D]
+        L.mgad_weno7_init.restype = I
+        L.mgad_weno7_rhs.argtypes = [VP, U, U, U, D, D, D, VP, VP]
+        L.mgad_weno7_rhs.restype = I
+        L.mgad_weno7_steps.argtypes = [VP, U, U, U, D, D, D, VP, VP, VP, VP, VP, I, I]
+        L.mgad_weno7_steps.restype = I
+        L.mgad_weno7_read_ctrl.argtypes = [VP, VP, ctypes.POINTER(D), ctypes.POINTER(D),
+                                           ctypes.POINTER(ctypes.c_ulonglong)]
+        L.mgad_weno7_read_ctrl.restype = I
         L.mgad_check_division.argtypes = [VP, I, VP, VP, VP, VP, ctypes.c_size_t]
         L.mgad_check_division.restype = I
         L.mgad_save_binary3d.argtypes = [VP, U, U, U, ctypes.c_char_p]

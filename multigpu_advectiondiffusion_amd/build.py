@@ -18,7 +18,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libmgad.so")
 ARCH = os.environ.get("MGAD_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["burgers_periodic.hip", "burgers_ref.hip", "batch.hip", "adjoint.hip", "dropin.hip", "heat.hip", "selfcheck.hip", "slab.hip",
+SOURCES = ["burgers_periodic.hip", "burgers_ref.hip", "batch.hip", "adjoint.hip", "dropin.hip", "heat.hip", "selfcheck.hip", "slab.hip", "weno7.hip",
            "host.cpp", "io.cpp"]
 # Per-file changes to HIPCC_FLAGS: the adjoint is checked to a tolerance, not
 # bit for bit, so it lets the compiler fuse multiply-adds.

@@ -1,0 +1,472 @@
+// weno7.hip -- the 3-D inviscid Burgers prototype with WENO7
+// (Matlab_Prototipes/InviscidBurgersNd/LFWENO7FDM3d.m, WENO7resAdv_X/Y/Z.m):
+// local Lax-Friedrichs split, seventh-order WENO faces with the Balsara-Shu
+// smoothness forms, ends extended by repeating the boundary value (3 nodes
+// before, 4 after), the face below node 0 of each line taken as 0, and the
+// low-storage 5-stage 4th-order Runge-Kutta of LFWENO7FDM3d.m:66-108.
+//
+// One launch per RK stage: for every node the right-hand side
+//   L = (dF + dG) + dH,  dF = (((hp+hn)_{i+1/2} - hp_{i-1/2}) - hn_{i-1/2})/dx
+// then res = a_s*res + dt*L and u_out = u_in - b_s*res.  The structure is the
+// periodic WENO5 stage kernel's: a TX x TY column tile marches a chunk of z
+// planes with an 8-plane register queue of split fluxes for the z faces; each
+// plane's split fluxes (tile + 4-node halo) go through LDS for the x and y
+// faces; every face is computed once.  Out-of-grid indices clamp to the ends,
+// which is the prototype's extension.  Time step control lives on the device
+// (dt = cfl*dx/max|u| of the step's start, clipped to t_end).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <map>
+#include <tuple>
+
+#include "burgers_periodic.hpp"
+
+namespace mgad {
+
+// The prototype's split (WENO7resAdv_X.m:8-10, flux = w.^2/2, dflux = w):
+// V = 0.5*(w^2/2 + |w| w) = f+, U = 0.5*(w^2/2 - |w| w) = f-.
+__device__ __forceinline__ void w7_split(double w, double &V, double &U) {
+    const double f = (w * w) / 2, a = fabs(w) * w;
+    V = 0.5 * (f + a);
+    U = 0.5 * (f - a);
+}
+
+// The four smoothness forms over a 7-node window s0..s6, in the source's
+// expression order (WENO7resAdv_X.m:25-32, :58-65).
+__device__ __forceinline__ void w7_beta(double a, double b, double c, double d, double e, double f, double g,
+                                        double B[4]) {
+    B[0] = c * (134241 * c - 114894 * d) + a * (56694 * c - 47214 * b + 6649 * a - 22778 * d) + 25729 * (d * d) +
+           b * (-210282 * c + 85641 * b + 86214 * d);
+    B[1] = d * (41001 * d - 30414 * e) + b * (-19374 * c + 3169 * b + 19014 * d - 5978 * e) + 6649 * (e * e) +
+           c * (33441 * c - 70602 * d + 23094 * e);
+    B[2] = e * (33441 * e - 19374 * f) + c * (6649 * c - 30414 * d + 23094 * e - 5978 * f) + 3169 * (f * f) +
+           d * (41001 * d - 70602 * e + 19014 * f);
+    B[3] = f * (85641 * f - 47214 * g) + d * (25729 * d - 114894 * e + 86214 * f - 22778 * g) + 6649 * (g * g) +
+           e * (134241 * e - 210282 * f + 56694 * g);
+}
+
+// w_k = alpha_k / sum, alpha_k = g_k/(eps + B_k)^2, eps = 1e-6 (:35-47).
+__device__ __forceinline__ void w7_weights(const double B[4], double g0, double g1, double g2, double g3,
+                                           double w[4]) {
+    const double eps = 1e-6;
+    const double a0 = g0 / ((eps + B[0]) * (eps + B[0])), a1 = g1 / ((eps + B[1]) * (eps + B[1])),
+                 a2 = g2 / ((eps + B[2]) * (eps + B[2])), a3 = g3 / ((eps + B[3]) * (eps + B[3]));
+    const double sum = a0 + a1 + a2 + a3;
+    w[0] = a0 / sum;
+    w[1] = a1 / sum;
+    w[2] = a2 / sum;
+    w[3] = a3 / sum;
+}
+
+constexpr double kR12 = 1.0 / 12;  // RN(1/12) for div_by
+
+// (hn, hp) at face i+1/2 from V = f+ and U = f- of nodes i-3..i+4 (v[0..7],
+// u[0..7]); candidates (w_k * poly_k)/12 summed left to right (:42-45, :82-85).
+__device__ __forceinline__ double2 w7_face(const double v[8], const double uu[8]) {
+    double B[4], w[4];
+    w7_beta(v[0], v[1], v[2], v[3], v[4], v[5], v[6], B);
+    w7_weights(B, 1. / 35, 12. / 35, 18. / 35, 4. / 35, w);
+    const double hn = div_by(w[0] * (-3 * v[0] + 13 * v[1] - 23 * v[2] + 25 * v[3]), 12.0, kR12) +
+                      div_by(w[1] * (1 * v[1] - 5 * v[2] + 13 * v[3] + 3 * v[4]), 12.0, kR12) +
+                      div_by(w[2] * (-1 * v[2] + 7 * v[3] + 7 * v[4] - 1 * v[5]), 12.0, kR12) +
+                      div_by(w[3] * (3 * v[3] + 13 * v[4] - 5 * v[5] + 1 * v[6]), 12.0, kR12);
+    const double *u = uu + 1;  // nodes i-2..i+4
+    w7_beta(u[0], u[1], u[2], u[3], u[4], u[5], u[6], B);
+    w7_weights(B, 4. / 35, 18. / 35, 12. / 35, 1. / 35, w);
+    const double hp = div_by(w[0] * (1 * u[0] - 5 * u[1] + 13 * u[2] + 3 * u[3]), 12.0, kR12) +
+                      div_by(w[1] * (-1 * u[1] + 7 * u[2] + 7 * u[3] - 1 * u[4]), 12.0, kR12) +
+                      div_by(w[2] * (3 * u[2] + 13 * u[3] - 5 * u[4] + 1 * u[5]), 12.0, kR12) +
+                      div_by(w[3] * (25 * u[3] - 23 * u[4] + 13 * u[5] - 3 * u[6]), 12.0, kR12);
+    return make_double2(hn, hp);
+}
+
+// The 5-stage scheme's coefficients (LFWENO7FDM3d.m:66-77), each the
+// correctly rounded quotient of the two published integers.
+__host__ __device__ constexpr double lsrk_a(int s) {
+    return s == 0   ? 0.0
+           : s == 1 ? -567301805773.0 / 1357537059087.0
+           : s == 2 ? -2404267990393.0 / 2016746695238.0
+           : s == 3 ? -3550918686646.0 / 2091501179385.0
+                    : -1275806237668.0 / 842570457699.0;
+}
+__host__ __device__ constexpr double lsrk_b(int s) {
+    return s == 0   ? 1432997174477.0 / 9575080441755.0
+           : s == 1 ? 5161836677717.0 / 13612068292357.0
+           : s == 2 ? 1720146321549.0 / 2090206949498.0
+           : s == 3 ? 3134564353537.0 / 4481467310338.0
+                    : 2277821191437.0 / 14882151754819.0;
+}
+
+// Device step control: stage 0 derives dt from max|u| of the step's start
+// (every block computes the same value) and the lead records it, or 0 once
+// t >= tend (then every stage of the step is a no-op); stage 1's lead moves t
+// and the step count forward and clears the max|u| slot stage 4 fills for
+// the next step.
+struct W7Ctrl {
+    double t, dt, tend, cfl;
+    unsigned long long amax[2];
+    unsigned long long steps;
+};
+static_assert(sizeof(W7Ctrl) <= MGAD_CTRL_BYTES, "ctrl layout");
+
+__device__ __forceinline__ bool w7_control(W7Ctrl *c, int s, double dx, bool lead, double &dt) {
+    if (s == 0) {
+        const double t = c->t, tend = c->tend;
+        if (!(t < tend)) {
+            if (lead) c->dt = 0.0;
+            return false;
+        }
+        double d = c->cfl * dx / bits_double(c->amax[c->steps & 1]);
+        if (t + d > tend) d = tend - t;
+        dt = d;
+        if (lead) c->dt = d;
+        return true;
+    }
+    dt = c->dt;
+    if (!(dt > 0.0)) return false;
+    if (s == 1 && lead) {
+        c->t = c->t + dt;
+        const unsigned long long n = c->steps + 1;
+        c->steps = n;
+        c->amax[n & 1] = 0ull;
+    }
+    return true;
+}
+
+struct W7Args {
+    int nx, ny, nz, kchunk, ntx, nty;
+    double dx, dy, dz, rdx, rdy, rdz;
+};
+
+template <int TX, int TY>
+struct W7Smem {
+    double2 P[TY + 8][TX + 8];  // (V, U) of the current plane, tile + 4-node halo
+    double2 FX[TY][TX + 1];     // (hn, hp) of x faces x0-1/2 .. x0+TX-1/2
+    double2 FY[TY + 1][TX];
+    unsigned long long red[TX * TY / 64];
+};
+
+// MODE 0: one LSRK stage s (res and u_out updated); MODE 1: res = L only.
+template <int TX, int TY, int MODE>
+__global__ __launch_bounds__(TX * TY, 2) void k_weno7_stage(const double *__restrict__ uin, double *uout,
+                                                          double *res, W7Args g, int s, double a, double b,
+                                                          W7Ctrl *ctrl) {
+    constexpr int NT = TX * TY, NW = NT / 64;
+    static_assert(NT % 64 == 0 && 8 * TX + 8 * TY <= NT && TX + TY <= 64, "tile shape");
+    __shared__ W7Smem<TX, TY> sm;
+    double dt = 0.0;
+    if (MODE == 0 && !w7_control(ctrl, s, g.dx, blockIdx.x == 0 && threadIdx.x == 0, dt)) return;
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int bx = tile % g.ntx, by = (tile / g.ntx) % g.nty, bz = tile / (g.ntx * g.nty);
+    const int x0 = bx * TX, y0 = by * TY, z0 = bz * g.kchunk, z1 = min(z0 + g.kchunk, nz);
+    const int tid = threadIdx.x, tx = tid % TX, ty = tid / TX;
+    const int wave = tid >> 6, lane = tid & 63;
+    const bool active = x0 + tx < nx && y0 + ty < ny;
+    const unsigned plane = static_cast<unsigned>(nx) * static_cast<unsigned>(ny);
+    auto cx = [&](int x) { return static_cast<unsigned>(min(max(x, 0), nx - 1)); };
+    auto cy = [&](int y) { return static_cast<unsigned>(min(max(y, 0), ny - 1)); };
+    auto cz = [&](int z) { return static_cast<unsigned>(min(max(z, 0), nz - 1)); };
+    const unsigned col = cx(x0 + tx) + static_cast<unsigned>(nx) * cy(y0 + ty);
+
+    // halo node of this thread: 8*TY x-halo nodes, then 8*TX y-halo nodes
+    int hl_r = 0, hl_c = 0;
+    unsigned hcol = 0;
+    const bool has_halo = tid < 8 * TY + 8 * TX;
+    if (tid < 8 * TY) {
+        const int r = tid / 8, c = tid % 8;
+        hl_r = r + 4;
+        hl_c = c < 4 ? c : TX + c;
+        hcol = cx(x0 + hl_c - 4) + static_cast<unsigned>(nx) * cy(y0 + r);
+    } else if (has_halo) {
+        const int h = tid - 8 * TY, c = h % TX, q = h / TX;
+        hl_r = q < 4 ? q : TY + q;
+        hl_c = c + 4;
+        hcol = cx(x0 + c) + static_cast<unsigned>(nx) * cy(y0 + hl_r - 4);
+    }
+
+    // z queue: qv/qu[q] hold V, U of plane k-3+q; u of plane k is qw
+    double qv[8], qu[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) w7_split(uin[col + plane * cz(z0 - 4 + q)], qv[q], qu[q]);
+    double2 fz_prev = z0 == 0 ? make_double2(0.0, 0.0) : w7_face(qv, qu);
+    double own_next = uin[col + plane * cz(z0 + 4)];
+    double halo_next = has_halo ? uin[hcol + plane * static_cast<unsigned>(z0)] : 0.0;
+    double uc_next = uin[col + plane * static_cast<unsigned>(z0)];
+    double res_next = MODE == 0 ? res[col + plane * static_cast<unsigned>(z0)] : 0.0;
+    unsigned long long vmax = 0ull;
+
+    for (int k = z0; k < z1; k++) {
+        const double own = own_next, halo = halo_next, uc = uc_next, rv = res_next;
+        if (k + 1 < z1) {
+            own_next = uin[col + plane * cz(k + 5)];
+            if (has_halo) halo_next = uin[hcol + plane * static_cast<unsigned>(k + 1)];
+            uc_next = uin[col + plane * static_cast<unsigned>(k + 1)];
+            if (MODE == 0) res_next = res[col + plane * static_cast<unsigned>(k + 1)];
+        }
+#pragma unroll
+        for (int q = 0; q < 7; q++) {
+            qv[q] = qv[q + 1];
+            qu[q] = qu[q + 1];
+        }
+        w7_split(own, qv[7], qu[7]);
+        {
+            double hv, hu;
+            w7_split(uc, hv, hu);
+            sm.P[ty + 4][tx + 4] = make_double2(hv, hu);
+        }
+        if (has_halo) {
+            double hv, hu;
+            w7_split(halo, hv, hu);
+            sm.P[hl_r][hl_c] = make_double2(hv, hu);
+        }
+        __syncthreads();
+
+        const double2 fz = w7_face(qv, qu);
+        {
+            double v[8], u[8];
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                const double2 p = sm.P[ty + 4][tx + 1 + q];
+                v[q] = p.x;
+                u[q] = p.y;
+            }
+            sm.FX[ty][tx + 1] = w7_face(v, u);
+        }
+        {
+            double v[8], u[8];
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                const double2 p = sm.P[ty + 1 + q][tx + 4];
+                v[q] = p.x;
+                u[q] = p.y;
+            }
+            sm.FY[ty + 1][tx] = w7_face(v, u);
+        }
+        // faces on the tile's low edges (zero below node 0), one wave rotating with k
+        if (wave == k % NW && lane < TY + TX) {
+            double v[8], u[8];
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                const double2 p = lane < TY ? sm.P[lane + 4][q] : sm.P[q][lane - TY + 4];
+                v[q] = p.x;
+                u[q] = p.y;
+            }
+            const bool at_start = lane < TY ? x0 == 0 : y0 == 0;
+            const double2 f = at_start ? make_double2(0.0, 0.0) : w7_face(v, u);
+            if (lane < TY)
+                sm.FX[lane][0] = f;
+            else
+                sm.FY[0][lane - TY] = f;
+        }
+        __syncthreads();
+
+        const double2 xr = sm.FX[ty][tx + 1], xl = sm.FX[ty][tx];
+        const double2 yr = sm.FY[ty + 1][tx], yl = sm.FY[ty][tx];
+        const double dF = div_by(xr.y + xr.x - xl.y - xl.x, g.dx, g.rdx);
+        const double dG = div_by(yr.y + yr.x - yl.y - yl.x, g.dy, g.rdy);
+        const double dH = div_by(fz.y + fz.x - fz_prev.y - fz_prev.x, g.dz, g.rdz);
+        const double L = dF + dG + dH;
+        const unsigned o = col + plane * static_cast<unsigned>(k);
+        if (MODE == 0) {
+            const double rn = a * rv + dt * L;
+            const double un = uc - b * rn;
+            if (active) {
+                res[o] = rn;
+                uout[o] = un;
+                const unsigned long long bb = abs_bits(un);
+                vmax = bb > vmax ? bb : vmax;
+            }
+        } else if (active) {
+            res[o] = L;
+        }
+        fz_prev = fz;
+    }
+
+    if (MODE == 0 && s == 4) {
+        vmax = wave_max_u64(vmax);
+        if (lane == 0) sm.red[wave] = vmax;
+        __syncthreads();
+        if (tid == 0) {
+            unsigned long long m = sm.red[0];
+#pragma unroll
+            for (int w = 1; w < NW; w++) m = sm.red[w] > m ? sm.red[w] : m;
+            atomicMax(&ctrl->amax[ctrl->steps & 1], m);
+        }
+    }
+}
+
+__global__ void k_w7_ctrl_init(W7Ctrl *c, double t0, double tend, double cfl) {
+    c->t = t0;
+    c->dt = 0.0;
+    c->tend = tend;
+    c->cfl = cfl;
+    c->amax[0] = 0ull;
+    c->amax[1] = 0ull;
+    c->steps = 0ull;
+}
+
+__global__ __launch_bounds__(256) void k_w7_prepare(const double *__restrict__ u, double *res, size_t n,
+                                                    unsigned long long *slot) {
+    __shared__ unsigned long long red[4];
+    unsigned long long m = 0ull;
+    for (size_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += 256ull * gridDim.x) {
+        res[i] = 0.0;
+        const unsigned long long b = abs_bits(u[i]);
+        m = b > m ? b : m;
+    }
+    m = wave_max_u64(m);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        m = red[0];
+        for (int w = 1; w < 4; w++) m = red[w] > m ? red[w] : m;
+        atomicMax(slot, m);
+    }
+}
+
+namespace {
+
+constexpr int kW7TX = 32, kW7TY = 16;
+
+bool w7_grid_ok(unsigned nx, unsigned ny, unsigned nz) {
+    return nx >= 1 && ny >= 1 && nz >= 1 && 1ull * nx * ny * nz < (1ull << 31);
+}
+
+W7Args w7_args(unsigned nx, unsigned ny, unsigned nz, double dx, double dy, double dz, unsigned &blocks) {
+    W7Args g;
+    g.nx = static_cast<int>(nx);
+    g.ny = static_cast<int>(ny);
+    g.nz = static_cast<int>(nz);
+    g.ntx = static_cast<int>((nx + kW7TX - 1) / kW7TX);
+    g.nty = static_cast<int>((ny + kW7TY - 1) / kW7TY);
+    const long tiles = static_cast<long>(g.ntx) * g.nty;
+    int kc = static_cast<int>(nz);
+    while (kc > 8 && tiles * ((nz + kc - 1) / kc) < 1024) kc = (kc + 1) / 2;
+    g.kchunk = kc;
+    blocks = static_cast<unsigned>(tiles * ((nz + kc - 1) / kc));
+    g.dx = dx;
+    g.dy = dy;
+    g.dz = dz;
+    g.rdx = 1.0 / dx;
+    g.rdy = 1.0 / dy;
+    g.rdz = 1.0 / dz;
+    return g;
+}
+
+int enqueue_w7_step(hipStream_t s, unsigned nx, unsigned ny, unsigned nz, double dx, double dy, double dz,
+                    double *u, double *w1, double *w2, double *res, W7Ctrl *ctrl) {
+    unsigned blocks = 0;
+    const W7Args g = w7_args(nx, ny, nz, dx, dy, dz, blocks);
+    const double *in[5] = {u, w1, w2, w1, w2};
+    double *out[5] = {w1, w2, w1, w2, u};
+    for (int st = 0; st < 5; st++) {
+        hipLaunchKernelGGL((k_weno7_stage<kW7TX, kW7TY, 0>), dim3(blocks), dim3(kW7TX * kW7TY), 0, s, in[st],
+                           out[st], res, g, st, lsrk_a(st), lsrk_b(st), ctrl);
+        const int r = check_launch();
+        if (r) return r;
+    }
+    return MGAD_OK;
+}
+
+}  // namespace
+}  // namespace mgad
+
+using namespace mgad;
+
+extern "C" int mgad_weno7_init(hipStream_t stream, unsigned int nx, unsigned int ny, unsigned int nz,
+                               const double *u, double *res, void *ctrl, double t0, double tend, double cfl) {
+    if (!u || !res || !ctrl || u == res || !w7_grid_ok(nx, ny, nz)) {
+        set_error(MGAD_ERR_INVALID_ARG);
+        return MGAD_ERR_INVALID_ARG;
+    }
+    W7Ctrl *c = static_cast<W7Ctrl *>(ctrl);
+    hipLaunchKernelGGL(k_w7_ctrl_init, dim3(1), dim3(1), 0, stream, c, t0, tend, cfl);
+    int st = check_launch();
+    if (st) return st;
+    const size_t n = 1ull * nx * ny * nz;
+    const unsigned nb = static_cast<unsigned>(std::min<size_t>((n + 255) / 256, 2048));
+    hipLaunchKernelGGL(k_w7_prepare, dim3(nb), dim3(256), 0, stream, u, res, n, &c->amax[0]);
+    return check_launch();
+}
+
+extern "C" int mgad_weno7_rhs(hipStream_t stream, unsigned int nx, unsigned int ny, unsigned int nz, double dx,
+                              double dy, double dz, const double *u, double *L) {
+    if (!u || !L || u == L || !w7_grid_ok(nx, ny, nz) || !(dx > 0) || !(dy > 0) || !(dz > 0)) {
+        set_error(MGAD_ERR_INVALID_ARG);
+        return MGAD_ERR_INVALID_ARG;
+    }
+    unsigned blocks = 0;
+    const W7Args g = w7_args(nx, ny, nz, dx, dy, dz, blocks);
+    hipLaunchKernelGGL((k_weno7_stage<kW7TX, kW7TY, 1>), dim3(blocks), dim3(kW7TX * kW7TY), 0, stream, u, nullptr,
+                       L, g, 0, 0.0, 0.0, nullptr);
+    return check_launch();
+}
+
+extern "C" int mgad_weno7_steps(hipStream_t stream, unsigned int nx, unsigned int ny, unsigned int nz, double dx,
+                                double dy, double dz, double *u, double *w1, double *w2, double *res, void *ctrl,
+                                int nsteps, int use_graph) {
+    if (!u || !w1 || !w2 || !res || !ctrl || u == w1 || u == w2 || w1 == w2 || res == u || res == w1 ||
+        res == w2 || nsteps < 0 || !w7_grid_ok(nx, ny, nz) || !(dx > 0) || !(dy > 0) || !(dz > 0)) {
+        set_error(MGAD_ERR_INVALID_ARG);
+        return MGAD_ERR_INVALID_ARG;
+    }
+    W7Ctrl *c = static_cast<W7Ctrl *>(ctrl);
+    if (!use_graph || stream == nullptr) {
+        for (int s = 0; s < nsteps; s++) {
+            const int st = enqueue_w7_step(stream, nx, ny, nz, dx, dy, dz, u, w1, w2, res, c);
+            if (st) return st;
+        }
+        return MGAD_OK;
+    }
+    using Key = std::tuple<hipStream_t, unsigned, unsigned, unsigned, double, double, double, double *, double *,
+                           double *, double *, void *>;
+    static std::map<Key, hipGraphExec_t> cache;
+    const Key key{stream, nx, ny, nz, dx, dy, dz, u, w1, w2, res, ctrl};
+    hipGraphExec_t exec = nullptr;
+    auto it = cache.find(key);
+    if (it == cache.end()) {
+        if (cache.size() > 16) {
+            for (auto &kv : cache) (void)hipGraphExecDestroy(kv.second);
+            cache.clear();
+        }
+        hipGraph_t gph = nullptr;
+        int st = check_hip(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        if (st) return st;
+        const int est = enqueue_w7_step(stream, nx, ny, nz, dx, dy, dz, u, w1, w2, res, c);
+        st = check_hip(hipStreamEndCapture(stream, &gph));
+        if (est) {
+            if (gph) (void)hipGraphDestroy(gph);
+            return est;
+        }
+        if (st) return st;
+        st = check_hip(hipGraphInstantiate(&exec, gph, nullptr, nullptr, 0));
+        (void)hipGraphDestroy(gph);
+        if (st) return st;
+        cache[key] = exec;
+    } else {
+        exec = it->second;
+    }
+    for (int s = 0; s < nsteps; s++) {
+        const int st = check_hip(hipGraphLaunch(exec, stream));
+        if (st) return st;
+    }
+    return MGAD_OK;
+}
+
+extern "C" int mgad_weno7_read_ctrl(hipStream_t stream, const void *ctrl, double *t, double *dt,
+                                    unsigned long long *steps) {
+    if (!ctrl) {
+        set_error(MGAD_ERR_INVALID_ARG);
+        return MGAD_ERR_INVALID_ARG;
+    }
+    W7Ctrl c;
+    int st = check_hip(hipMemcpyAsync(&c, ctrl, sizeof(c), hipMemcpyDeviceToHost, stream));
+    if (!st) st = check_hip(hipStreamSynchronize(stream));
+    if (st) return st;
+    if (t) *t = c.t;
+    if (dt) *dt = c.dt;
+    if (steps) *steps = c.steps;
+    return MGAD_OK;
+}

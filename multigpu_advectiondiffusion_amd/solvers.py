@@ -184,6 +184,58 @@ class SlabBurgers:
         return self.slabs()[:, 3:3 + self.nzs].reshape(self.nz, self.ny, self.nx)
 
 
+class Weno7Burgers:
+    """The WENO7 Burgers prototype (Matlab_Prototipes/InviscidBurgersNd/
+    LFWENO7FDM3d.m) on the device: node grid on [lo, hi]^3 with dx = (hi-lo)/(n-1),
+    local LF, WENO7 faces, repeated-end extension, low-storage RK45,
+    dt = cfl*dx/max|u| clipped to tend."""
+
+    def __init__(self, shape, cfl: float = 0.4, lo: float = -1.0, hi: float = 1.0,
+                 stream: Stream | None = None):
+        self.nx, self.ny, self.nz = (shape, shape, shape) if np.isscalar(shape) else shape
+        self.dx, self.dy, self.dz = ((hi - lo) / max(n - 1, 1) for n in (self.nx, self.ny, self.nz))
+        self.lo, self.cfl = lo, cfl
+        self.stream = stream if stream is not None else Stream()
+        shp = (self.nz, self.ny, self.nx)
+        self.u, self.w1, self.w2, self.res = (DeviceArray(shp) for _ in range(4))
+        self.ctrl = DeviceArray((_capi.CTRL_BYTES,), np.uint8)
+
+    def gaussian(self) -> np.ndarray:
+        """The prototype's initial field exp(-(x^2+y^2+z^2)/0.1) (LFWENO7FDM3d.m:53)."""
+        x = self.lo + self.dx * np.arange(self.nx)
+        y = self.lo + self.dy * np.arange(self.ny)
+        z = self.lo + self.dz * np.arange(self.nz)
+        return 1.0 * np.exp(-(x[None, None, :] ** 2 + y[None, :, None] ** 2 + z[:, None, None] ** 2) / 0.1)
+
+    def set_initial(self, u0: np.ndarray, t0: float = 0.0, tend: float = float("inf")) -> None:
+        self.u.upload(u0)
+        rc = _capi.lib().mgad_weno7_init(self.stream.handle, self.nx, self.ny, self.nz, self.u.ptr, self.res.ptr,
+                                         self.ctrl.ptr, t0, tend, self.cfl)
+        _capi.check(rc, "mgad_weno7_init")
+
+    def steps(self, nsteps: int, graph: bool = True) -> None:
+        rc = _capi.lib().mgad_weno7_steps(self.stream.handle, self.nx, self.ny, self.nz, self.dx, self.dy, self.dz,
+                                          self.u.ptr, self.w1.ptr, self.w2.ptr, self.res.ptr, self.ctrl.ptr, nsteps,
+                                          1 if graph else 0)
+        _capi.check(rc, "mgad_weno7_steps")
+
+    def rhs(self, u: DeviceArray, out: DeviceArray) -> None:
+        rc = _capi.lib().mgad_weno7_rhs(self.stream.handle, self.nx, self.ny, self.nz, self.dx, self.dy, self.dz,
+                                        u.ptr, out.ptr)
+        _capi.check(rc, "mgad_weno7_rhs")
+
+    def state(self):
+        t, dt, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_ulonglong()
+        rc = _capi.lib().mgad_weno7_read_ctrl(self.stream.handle, self.ctrl.ptr, ctypes.byref(t), ctypes.byref(dt),
+                                              ctypes.byref(n))
+        _capi.check(rc, "mgad_weno7_read_ctrl")
+        return t.value, dt.value, n.value
+
+    def solution(self) -> np.ndarray:
+        self.stream.synchronize()
+        return self.u.download()
+
+
 class BurgersBatch:
     """Independent periodic Burgers problems of uneven cubic sizes advanced
     together (one persistent launch per SSP-RK3 stage), each with its own dt."""

@@ -80,6 +80,11 @@ def lib():
         L.orc_lu3d_13.argtypes = [_U, _U, _U, _D, _D, _D, _P, _P]
         L.orc_lu3d_o2.argtypes = [_U, _U, _U, _D, _D, _D, _P, _P]
         L.orc_rk3d.argtypes = [_U, _U, _U, _U, _D, _P, _P, _P]
+        L.orc_weno7_face.argtypes = [_P, _P, _P, _P]
+        L.orc_weno7_rhs.argtypes = [_U, _U, _U, _D, _D, _D, _P, _P]
+        L.orc_lsrk_stage.argtypes = [ctypes.c_uint64, ctypes.c_int, _D, _P, _P, _P]
+        L.orc_weno7_run.restype = ctypes.c_int
+        L.orc_weno7_run.argtypes = [_U, _U, _U, _D, _D, _D, _D, _P, _D, ctypes.c_int, _P, _P]
         L.orc_mb3_adv.argtypes = [ctypes.c_int, _U, _U, _U, _U, _U, _U, _D, _P, _P, _S]
         L.orc_mb3_rk.argtypes = [_U, _U, _U, _U, _U, _D, _P, _P, _P]
         L.orc_maxabs.restype = _D
@@ -172,6 +177,38 @@ def mb3_adv(axis: int, u: np.ndarray, Lu: np.ndarray, kstart: int, kstop: int, d
 def mb3_rk(q, qo, Lq, step, dt, nx: int | None = None) -> None:
     nz, ny, px = q.shape
     lib().orc_mb3_rk(step, px, px if nx is None else nx, ny, nz, dt, _p(q), _p(qo), _p(Lq))
+
+
+# ---- WENO7 Burgers prototype (Matlab_Prototipes/InviscidBurgersNd) ----
+
+def weno7_face(V, U):
+    """(hn, hp) at face i+1/2 from the split fluxes V = f+, U = f- of nodes i-3..i+4."""
+    V = np.ascontiguousarray(V, dtype=np.float64)
+    U = np.ascontiguousarray(U, dtype=np.float64)
+    assert V.size == 8 and U.size == 8
+    hn, hp = ctypes.c_double(), ctypes.c_double()
+    lib().orc_weno7_face(_p(V), _p(U), ctypes.byref(hn), ctypes.byref(hp))
+    return hn.value, hp.value
+
+
+def weno7_rhs(u, d) -> np.ndarray:
+    """dF + dG + dH of LFWENO7FDM3d.m (node grid, ends extrapolated)."""
+    dx, dy, dz = (d, d, d) if np.isscalar(d) else d
+    res = np.empty_like(u)
+    lib().orc_weno7_rhs(*_shape(u), dx, dy, dz, _p(u), _p(res))
+    return res
+
+
+def lsrk_stage(u, res, L, s, dt) -> None:
+    lib().orc_lsrk_stage(u.size, s, dt, _p(u), _p(res), _p(L))
+
+
+def weno7_run(u, res, d, cfl, t, tend, nsteps=-1):
+    """Advance u (and the carried residual res) in place; returns (steps, t)."""
+    dx, dy, dz = (d, d, d) if np.isscalar(d) else d
+    tt = ctypes.c_double(t)
+    n = lib().orc_weno7_run(*_shape(u), dx, dy, dz, cfl, ctypes.byref(tt), tend, nsteps, _p(u), _p(res))
+    return n, tt.value
 
 
 def diff13_b5(u, Lu, kx, ky, kz) -> None:

@@ -27,6 +27,16 @@
  *  - orc_mb3_adv, orc_mb3_rk ........ x/y on planes 3..nz-4 with pitch nx equal orc_adv_x/y;
  *                                       z equals orc_adv_z on planes 3..nz-4 and splits over
  *                                       plane ranges; RK equals rk_node on every node.
+ *  - orc_weno7_face ................. constant data gives hn = f+(c), hp = f-(c); every
+ *                                       smoothness form vanishes on constants and its
+ *                                       coefficients are a positive semidefinite form;
+ *                                       cubic data makes the four candidates agree; mirror
+ *                                       symmetry of the two halves; 7th-order convergence.
+ *  - orc_weno7_rhs .................. constant field: 0 except node 0 of each line
+ *                                       (face below it taken as 0); telescoping.
+ *  - orc_lsrk_stage, orc_weno7_run .. u' = lambda*u converges at 4th order; the stage
+ *                                       coefficients give b-sums of the classical
+ *                                       conditions.
  *  - orc_ssprk_b5, orc_rk3d,
  *    orc_rk_stage_all ............... linear ODE u' = lambda*u gives the SSP-RK3
  *                                       amplification 1 + z + z^2/2 + z^3/6.
@@ -407,6 +417,147 @@ void orc_mb3_rk(unsigned step, unsigned pitch, unsigned nx, unsigned ny, unsigne
                 size_t o = i + (size_t)pitch * j + (size_t)pitch * ny * k;
                 q[o] = rk_node(step, dt, q[o], qo[o], Lq[o]);
             }
+}
+
+/* ------------------------------------------------------------------------ */
+/* WENO7 Burgers prototype (Matlab_Prototipes/InviscidBurgersNd)             */
+/* ------------------------------------------------------------------------ */
+
+/* Smoothness indicators of the four 4-point stencils of a 7-point window
+ * s[0..6] (the Balsara-Shu quadratic forms), written as WENO7resAdv_X.m:
+ * 25-32 writes them for v(i-3)..v(i+3); the U side (:58-65) applies the same
+ * forms to u(i-2)..u(i+4). */
+static void w7_beta(const double *s, double *B) {
+    const double a = s[0], b = s[1], c = s[2], d = s[3], e = s[4], f = s[5], g = s[6];
+    B[0] = c * (134241 * c - 114894 * d) + a * (56694 * c - 47214 * b + 6649 * a - 22778 * d) +
+           25729 * (d * d) + b * (-210282 * c + 85641 * b + 86214 * d);
+    B[1] = d * (41001 * d - 30414 * e) + b * (-19374 * c + 3169 * b + 19014 * d - 5978 * e) +
+           6649 * (e * e) + c * (33441 * c - 70602 * d + 23094 * e);
+    B[2] = e * (33441 * e - 19374 * f) + c * (6649 * c - 30414 * d + 23094 * e - 5978 * f) +
+           3169 * (f * f) + d * (41001 * d - 70602 * e + 19014 * f);
+    B[3] = f * (85641 * f - 47214 * g) + d * (25729 * d - 114894 * e + 86214 * f - 22778 * g) +
+           6649 * (g * g) + e * (134241 * e - 210282 * f + 56694 * g);
+}
+
+/* Nonlinear weights alpha_k = g_k/(eps + B_k)^2, w_k = alpha_k/sum
+ * (WENO7resAdv_X.m:35-47, :68-80), eps = 1e-6. */
+static void w7_weights(const double *B, const double *g, double *w) {
+    const double eps = 1e-6;
+    double al[4];
+    for (int k = 0; k < 4; k++) al[k] = g[k] / ((eps + B[k]) * (eps + B[k]));
+    const double sum = al[0] + al[1] + al[2] + al[3];
+    for (int k = 0; k < 4; k++) w[k] = al[k] / sum;
+}
+
+/* The two halves of the face flux at i+1/2 from the split fluxes of nodes
+ * i-3..i+4: hn from V = f+ at i-3..i+3 (WENO7resAdv_X.m:14-52), hp from
+ * U = f- at i-2..i+4 (:55-85); candidate terms (w_k * poly_k)/12 summed left
+ * to right. */
+void orc_weno7_face(const double *V, const double *U, double *hn, double *hp) {
+    static const double gn[4] = {1. / 35, 12. / 35, 18. / 35, 4. / 35};
+    static const double gp[4] = {4. / 35, 18. / 35, 12. / 35, 1. / 35};
+    double B[4], w[4];
+    const double *v = V, *u = U + 1;  /* v: i-3..i+3, u: i-2..i+4 */
+    w7_beta(v, B);
+    w7_weights(B, gn, w);
+    *hn = w[0] * (-3 * v[0] + 13 * v[1] - 23 * v[2] + 25 * v[3]) / 12 +
+          w[1] * (1 * v[1] - 5 * v[2] + 13 * v[3] + 3 * v[4]) / 12 +
+          w[2] * (-1 * v[2] + 7 * v[3] + 7 * v[4] - 1 * v[5]) / 12 +
+          w[3] * (3 * v[3] + 13 * v[4] - 5 * v[5] + 1 * v[6]) / 12;
+    w7_beta(u, B);
+    w7_weights(B, gp, w);
+    *hp = w[0] * (1 * u[0] - 5 * u[1] + 13 * u[2] + 3 * u[3]) / 12 +
+          w[1] * (-1 * u[1] + 7 * u[2] + 7 * u[3] - 1 * u[4]) / 12 +
+          w[2] * (3 * u[2] + 13 * u[3] - 5 * u[4] + 1 * u[5]) / 12 +
+          w[3] * (25 * u[3] - 23 * u[4] + 13 * u[5] - 3 * u[6]) / 12;
+}
+
+/* Local LF split of the prototype (WENO7resAdv_X.m:8-10 with flux = w.^2/2,
+ * dflux = w): V = 0.5*(w^2/2 + |w| w), U = 0.5*(w^2/2 - |w| w). */
+static inline void w7_split(double w, double *V, double *U) {
+    const double f = (w * w) / 2, a = fabs(w);
+    *V = 0.5 * (f + a * w);
+    *U = 0.5 * (f - a * w);
+}
+
+/* df/dx of one line of n nodes (stride st), WENO7resAdv_X.m: ghost nodes
+ * repeat the end values (3 at the start, 4 at the end, :4), faces
+ * 1/2 .. n-1/2 are computed and the face below node 0 is 0 (hn, hp are zero
+ * outside I), res = (hp(I) + hn(I) - hp(I-1) - hn(I-1))/dx. */
+static void w7_line(const double *w, double *res, size_t base, size_t st, unsigned n, double d, int mode) {
+    double V[8], U[8];
+    double hn_old = 0.0, hp_old = 0.0;
+    for (unsigned i = 0; i < n; i++) {
+        for (int q = 0; q < 8; q++) {
+            long j = (long)i - 3 + q;
+            if (j < 0) j = 0;
+            if (j > (long)n - 1) j = (long)n - 1;
+            w7_split(w[base + st * (size_t)j], &V[q], &U[q]);
+        }
+        double hn, hp;
+        orc_weno7_face(V, U, &hn, &hp);
+        const double r = (hp + hn - hp_old - hn_old) / d;
+        if (mode == 0)
+            res[base + st * i] = r;
+        else
+            res[base + st * i] += r;
+        hn_old = hn;
+        hp_old = hp;
+    }
+}
+
+/* dF + dG + dH of LFWENO7FDM3d.m:101-103 (x, then + y, then + z). */
+void orc_weno7_rhs(unsigned nx, unsigned ny, unsigned nz, double dx, double dy, double dz,
+                   const double *u, double *res) {
+    const size_t XY = (size_t)nx * ny;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (unsigned k = 0; k < nz; k++)
+        for (unsigned j = 0; j < ny; j++) w7_line(u, res, (size_t)nx * j + XY * k, 1, nx, dx, 0);
+#pragma omp parallel for collapse(2) schedule(static)
+    for (unsigned k = 0; k < nz; k++)
+        for (unsigned i = 0; i < nx; i++) w7_line(u, res, i + XY * k, nx, ny, dy, 1);
+#pragma omp parallel for collapse(2) schedule(static)
+    for (unsigned j = 0; j < ny; j++)
+        for (unsigned i = 0; i < nx; i++) w7_line(u, res, i + (size_t)nx * j, XY, nz, dz, 1);
+}
+
+/* Low-storage 5-stage 4th-order Runge-Kutta of LFWENO7FDM3d.m:66-82. */
+const double orc_lsrk_a[5] = {0.0, -567301805773.0 / 1357537059087.0, -2404267990393.0 / 2016746695238.0,
+                              -3550918686646.0 / 2091501179385.0, -1275806237668.0 / 842570457699.0};
+const double orc_lsrk_b[5] = {1432997174477.0 / 9575080441755.0, 5161836677717.0 / 13612068292357.0,
+                              1720146321549.0 / 2090206949498.0, 3134564353537.0 / 4481467310338.0,
+                              2277821191437.0 / 14882151754819.0};
+
+/* Stage s (0..4) at one node: res = a_s*res + dt*L; u = u - b_s*res
+ * (LFWENO7FDM3d.m:104-105). */
+void orc_lsrk_stage(uint64_t n, int s, double dt, double *u, double *res, const double *L) {
+#pragma omp parallel for schedule(static)
+    for (uint64_t i = 0; i < n; i++) {
+        res[i] = orc_lsrk_a[s] * res[i] + dt * L[i];
+        u[i] = u[i] - orc_lsrk_b[s] * res[i];
+    }
+}
+
+/* The time loop of LFWENO7FDM3d.m:88-108: dt = cfl*dx/max|u| clipped to tend,
+ * five stages per step; res carries over between steps as there.  Advances
+ * u and res in place; returns the steps taken and leaves the time in *t. */
+int orc_weno7_run(unsigned nx, unsigned ny, unsigned nz, double dx, double dy, double dz, double cfl,
+                  double *t, double tend, int nsteps, double *u, double *res) {
+    const size_t N = (size_t)nx * ny * nz;
+    double *L = (double *)malloc(sizeof(double) * N);
+    int it = 0;
+    while (*t < tend && (nsteps < 0 || it < nsteps)) {
+        double dt = cfl * dx / orc_maxabs(N, u);
+        if (*t + dt > tend) dt = tend - *t;
+        *t = *t + dt;
+        for (int s = 0; s < 5; s++) {
+            orc_weno7_rhs(nx, ny, nz, dx, dy, dz, u, L);
+            orc_lsrk_stage(N, s, dt, u, res, L);
+        }
+        it++;
+    }
+    free(L);
+    return it;
 }
 
 /* ------------------------------------------------------------------------ */

@@ -83,6 +83,17 @@ void orc_mb3_rk(unsigned step, unsigned pitch, unsigned nx, unsigned ny,
                 unsigned nz, double dt, double *q, const double *qo,
                 const double *Lq);
 
+/* ---- WENO7 Burgers prototype (Matlab_Prototipes/InviscidBurgersNd) ---- */
+void orc_weno7_face(const double *V, const double *U, double *hn, double *hp);
+void orc_weno7_rhs(unsigned nx, unsigned ny, unsigned nz, double dx, double dy,
+                   double dz, const double *u, double *res);
+extern const double orc_lsrk_a[5], orc_lsrk_b[5];
+void orc_lsrk_stage(uint64_t n, int s, double dt, double *u, double *res,
+                    const double *L);
+int orc_weno7_run(unsigned nx, unsigned ny, unsigned nz, double dx, double dy,
+                  double dz, double cfl, double *t, double tend, int nsteps,
+                  double *u, double *res);
+
 /* ---- periodic global-LF Burgers (the fused path's specification) ---- */
 double orc_maxabs(uint64_t n, const double *u);
 void   orc_rhs_periodic(unsigned nx, unsigned ny, unsigned nz, double dx,
