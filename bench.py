@@ -16,7 +16,8 @@ directions), the FP32-storage error against FP64 (512^3 to t = 0.5), the
 reference's own viscous Burgers runs with the fused reference step against
 the kernel times the reference published for them, and the z-slab form of
 the run (one 256x256x32 slab, one GPU's share of an 8-way 256^3
-decomposition; 8 slabs on one GPU) against the unsplit grid;
+decomposition; 8 slabs on one GPU) against the unsplit grid, and the WENO7
+prototype's scheme at 256^3;
 --quick skips them.  Progress goes to stderr.
 """
 from __future__ import annotations
@@ -199,6 +200,9 @@ def main() -> int:
         out["z_slabs"] = {k: sl[k] for k in ("one_slab_256x256x32_gpts", "eight_slabs_256_gpts", "unsplit_256_gpts",
                                              "one_slab_vs_unsplit", "eight_slabs_vs_unsplit")}
         print("slab study done", file=sys.stderr, flush=True)
+        w7 = studies.weno7_cases()
+        out["weno7_256"] = w7["grid_256"]
+        print("weno7 study done", file=sys.stderr, flush=True)
     print(json.dumps(out), flush=True)
     return 0
 

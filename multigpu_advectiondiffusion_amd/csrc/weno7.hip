@@ -149,7 +149,7 @@ struct W7Smem {
 
 // MODE 0: one LSRK stage s (res and u_out updated); MODE 1: res = L only.
 template <int TX, int TY, int MODE>
-__global__ __launch_bounds__(TX * TY, 2) void k_weno7_stage(const double *__restrict__ uin, double *uout,
+__global__ __launch_bounds__(TX * TY, 4) void k_weno7_stage(const double *__restrict__ uin, double *uout,
                                                           double *res, W7Args g, int s, double a, double b,
                                                           W7Ctrl *ctrl) {
     constexpr int NT = TX * TY, NW = NT / 64;
@@ -328,7 +328,9 @@ __global__ __launch_bounds__(256) void k_w7_prepare(const double *__restrict__ u
 
 namespace {
 
-constexpr int kW7TX = 32, kW7TY = 16;
+// 16 x 16 tiles at 4 blocks per CU: 3.41 Gpts/s at 256^3 against 3.23 for
+// 32 x 16 at 2 blocks per CU
+constexpr int kW7TX = 16, kW7TY = 16;
 
 bool w7_grid_ok(unsigned nx, unsigned ny, unsigned nz) {
     return nx >= 1 && ny >= 1 && nz >= 1 && 1ull * nx * ny * nz < (1ull << 31);

@@ -6,6 +6,7 @@
     python -m multigpu_advectiondiffusion_amd.studies reference
     python -m multigpu_advectiondiffusion_amd.studies heat
     python -m multigpu_advectiondiffusion_amd.studies slab
+    python -m multigpu_advectiondiffusion_amd.studies weno7
 
 batched: 64 independent periodic WENO5-Z (p=2, eps=1e-40) FP64 problems with
     edge lengths drawn seeded-uniformly from {32,48,64,96,128} (SplitMix64 seed
@@ -38,6 +39,10 @@ slab: the headline WENO5-Z (p=2) FP64 run on z slabs with 3 ghost planes:
     against the unsplit 256^3 run; each 30 hipGraph-replayed steps after 3
     warm-up steps.  The exchange copies stand in for the per-stage neighbour
     send/recv.
+weno7: the WENO7 Burgers prototype (LFWENO7FDM3d.m: Gaussian initial field,
+    CFL 0.4, low-storage RK45) at 256^3 FP64, 10 steps after 2 warm-up steps,
+    hipGraph-replayed; the prototype's own 100^3 run to tEnd 0.4 with its step
+    count and wall time.
 Each prints one JSON line.
 """
 from __future__ import annotations
@@ -401,9 +406,37 @@ def slab_cases(steps: int = 30, warmup: int = 3) -> dict:
     return out
 
 
+def weno7_cases(steps: int = 10, warmup: int = 2) -> dict:
+    e0, e1 = hip.Event(), hip.Event()
+    out = {"study": "weno7", "scheme": "WENO7 (Balsara-Shu forms, eps 1e-6), local LF, LSRK(5,4)"}
+    s = solvers.Weno7Burgers(256)
+    s.set_initial(s.gaussian())
+    s.steps(warmup)
+    hip.synchronize()
+    e0.record(s.stream)
+    s.steps(steps)
+    e1.record(s.stream)
+    e1.synchronize()
+    t = e0.elapsed_ms(e1) / 1e3
+    out["grid_256"] = {"steps": steps, "gpts": 256 ** 3 * steps / t / 1e9, "ms_per_step": 1e3 * t / steps,
+                       "stage_launches_per_step": 5}
+    del s
+    s = solvers.Weno7Burgers(100, cfl=0.4)
+    s.set_initial(s.gaussian(), tend=0.4)
+    hip.synchronize()
+    e0.record(s.stream)
+    s.steps(400)  # more than the run needs; steps after tEnd are no-ops on the device
+    e1.record(s.stream)
+    e1.synchronize()
+    tt, dt, n = s.state()
+    out["prototype_100_to_0.4"] = {"steps": n, "t": tt, "wall_s": e0.elapsed_ms(e1) / 1e3,
+                                   "max_u": float(np.max(np.abs(s.solution())))}
+    return out
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
-    ap.add_argument("study", choices=["batched", "fp32", "adjoint", "reference", "heat", "slab"])
+    ap.add_argument("study", choices=["batched", "fp32", "adjoint", "reference", "heat", "slab", "weno7"])
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--n", type=int, default=512)
     args = ap.parse_args(argv)
@@ -415,6 +448,8 @@ def main(argv=None) -> int:
         r = heat_cases()
     elif args.study == "slab":
         r = slab_cases()
+    elif args.study == "weno7":
+        r = weno7_cases()
     elif args.study == "adjoint":
         r = adjoint(args.n if args.n != 512 else 128)
     else:

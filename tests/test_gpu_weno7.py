@@ -12,7 +12,7 @@ from multigpu_advectiondiffusion_amd.hip import DeviceArray
 pytestmark = pytest.mark.gpu
 
 
-# partial tiles in x and y (tiles are 32 x 16), single-node and thin axes
+# partial tiles in x and y (tiles are 16 x 16), single-node and thin axes
 @pytest.mark.parametrize("shape", [(20, 20, 20), (37, 19, 11), (70, 33, 5), (1, 7, 9), (64, 48, 40)])
 def test_rhs_bitwise(hipdev, shape):
     s = solvers.Weno7Burgers(shape)
