@@ -259,6 +259,9 @@ __global__ __launch_bounds__(TX * TY, 3) void k_vjp_xy(
     static_assert(NT % 64 == 0 && TX >= 5 && TY >= 5, "tile shape");
     static_assert(5 * (TX + TY) + TX + TY <= NT, "one halo role per thread");
     __shared__ VjpSmem<TX, TY> sm;
+    // the argmax slot of the stage processed before this one (reverse order
+    // 3, 2, 1, 3, ...) has been used by k_vjp_tail: clear it for its next use
+    if (blockIdx.x == 0 && threadIdx.x == 0) sc->idx[stage == 3 ? 0 : stage] = 0xffffffffu;
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     const int ntx = nx / TX, nty = ny / TY;
     const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
@@ -413,7 +416,7 @@ __global__ __launch_bounds__(TX * TY, 3) void k_vjp_xy(
 // left for it (x offsets 0..2 and TX-2..TX-1 within its tile, the same in
 // y).  Node k-2 is finished once its last own face, k+1/2, is done;
 // contributions to nodes of the chunks below (z0-2, z0-1) and above
-// (z1..z1+2) go to exz, which k_vjp_zinflow adds.
+// (z1..z1+2) go to exz, which k_vjp_tail adds.
 template <int WENO, int P, int TX, int TY>
 __global__ __launch_bounds__(256, 4) void k_vjp_z(const double *__restrict__ u, const double *__restrict__ ubo,
                                                double *ubi, const StepRec *rec, int stage, VjpGrid g, double eps,
@@ -485,28 +488,44 @@ __global__ __launch_bounds__(256, 4) void k_vjp_z(const double *__restrict__ u, 
     block_partials<256>(sL, sLa, red, partials);
 }
 
-// Adds exz to the planes that received z contributions from a neighbouring
-// chunk: z0..z0+2 and z1-2..z1-1 of every chunk (5 planes per chunk, each
-// written by exactly one chunk).
-__global__ __launch_bounds__(256) void k_vjp_zinflow(double *ubi, const double *__restrict__ exz, VjpGrid g) {
-    const unsigned plane = static_cast<unsigned>(g.nx) * static_cast<unsigned>(g.ny);
-    const size_t n = static_cast<size_t>(plane) * 5 * g.nzc;
-    for (size_t i = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x; i < n; i += 256ull * gridDim.x) {
-        const unsigned col = static_cast<unsigned>(i % plane);
-        const int p = static_cast<int>(i / plane), bz = p / 5, j = p % 5;
-        const int z0 = range_lo(bz, g.nzc, g.nz), z1 = range_lo(bz + 1, g.nzc, g.nz);
-        const int z = j < 3 ? z0 + j : z1 - 5 + j;
-        const unsigned o = col + plane * static_cast<unsigned>(z);
-        ubi[o] += exz[o];
+// The end of a stage VJP, one launch:
+//  - block 0 sums the stage's partials (x, y, z blocks; each thread a fixed
+//    strided subset, then a fixed-order tree), routes the alpha adjoint to
+//    the argmax node and, at stage 1, closes the dt chain; being dispatched
+//    first, it runs alongside the others;
+//  - the other blocks add exz to the planes that received z contributions
+//    from a neighbouring chunk: z0..z0+2 and z1-2..z1-1 of every chunk (5
+//    planes per chunk, each written by exactly one chunk).
+// The argmax node is left to block 0, which adds its exz term first, so its
+// value is (ubi + exz) + alpha-term as with separate launches.  Its slot is
+// cleared by the next stage's k_vjp_xy, not here, since the other blocks
+// read it.
+__device__ __forceinline__ bool inflow_plane(int z, const VjpGrid &g) {
+    for (int c = 0; c < g.nzc; c++) {
+        const int z0 = range_lo(c, g.nzc, g.nz), z1 = range_lo(c + 1, g.nzc, g.nz);
+        if (z >= z0 && z < z1) return z < z0 + 3 || z >= z1 - 2;
     }
+    return false;
 }
 
-// One block: sums the stage's partials (x, y, z blocks; each thread a fixed
-// strided subset, then a fixed-order tree), routes the alpha adjoint to its
-// argmax node and, at stage 1, closes the dt chain.
-__global__ __launch_bounds__(256) void k_adj_scalars(const double *partials, int nparts, const StepRec *rec,
-                                                     int stage, double *ubi, const double *uin, AdjScalars *sc,
-                                                     int clipped_last) {
+__global__ __launch_bounds__(256) void k_vjp_tail(double *ubi, const double *__restrict__ exz, VjpGrid g,
+                                                  const double *partials, int nparts, const StepRec *rec, int stage,
+                                                  const double *uin, AdjScalars *sc, int clipped_last) {
+    const unsigned plane = static_cast<unsigned>(g.nx) * static_cast<unsigned>(g.ny);
+    const unsigned idx = sc->idx[stage - 1];
+    if (blockIdx.x > 0) {
+        const size_t n = static_cast<size_t>(plane) * 5 * g.nzc;
+        for (size_t i = static_cast<size_t>(blockIdx.x - 1) * 256 + threadIdx.x; i < n;
+             i += 256ull * (gridDim.x - 1)) {
+            const unsigned col = static_cast<unsigned>(i % plane);
+            const int p = static_cast<int>(i / plane), bz = p / 5, j = p % 5;
+            const int z0 = range_lo(bz, g.nzc, g.nz), z1 = range_lo(bz + 1, g.nzc, g.nz);
+            const int z = j < 3 ? z0 + j : z1 - 5 + j;
+            const unsigned o = col + plane * static_cast<unsigned>(z);
+            if (o != idx) ubi[o] += exz[o];
+        }
+        return;
+    }
     __shared__ double red[2][256];
     double L = 0.0, La = 0.0;
     for (int b = threadIdx.x; b < nparts; b += 256) {
@@ -537,9 +556,11 @@ __global__ __launch_bounds__(256) void k_adj_scalars(const double *partials, int
         }
         sc->dtb = 0.0;
     }
-    const unsigned idx = sc->idx[stage - 1];
-    if (idx != 0xffffffffu) ubi[idx] += ab * (uin[idx] >= 0.0 ? 1.0 : -1.0);
-    sc->idx[stage - 1] = 0xffffffffu;
+    if (idx != 0xffffffffu) {
+        double v = ubi[idx];
+        if (inflow_plane(static_cast<int>(idx / plane), g)) v += exz[idx];
+        ubi[idx] = v + ab * (uin[idx] >= 0.0 ? 1.0 : -1.0);
+    }
 }
 
 __global__ void k_adj_reset(AdjScalars *sc) {
@@ -658,10 +679,9 @@ int launch_stage_vjp(hipStream_t s, int stage, const double *uin, const double *
     hipLaunchKernelGGL((k_vjp_z<WENO, P, kVjpTX, kVjpTY>), dim3(nzb), dim3(256), 0, s, uin, ubo, ubi, rec, stage, g,
                        eps, exx, exy, exz, partials + 2 * nxy);
     const size_t nin = static_cast<size_t>(g.nx) * g.ny * 5 * g.nzc;
-    hipLaunchKernelGGL(k_vjp_zinflow, dim3(static_cast<unsigned>(std::min<size_t>((nin + 255) / 256, 8192))),
-                       dim3(256), 0, s, ubi, exz, g);
-    hipLaunchKernelGGL(k_adj_scalars, dim3(1), dim3(256), 0, s, partials, nxy + nzb, rec, stage, ubi, uin, sc,
-                       clipped_last ? 1 : 0);
+    const unsigned tail_blocks = static_cast<unsigned>(std::min<size_t>((nin + 255) / 256, 8192)) + 1;
+    hipLaunchKernelGGL(k_vjp_tail, dim3(tail_blocks), dim3(256), 0, s, ubi, exz, g, partials, nxy + nzb, rec, stage,
+                       uin, sc, clipped_last ? 1 : 0);
     return check_launch();
 }
 
