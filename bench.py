@@ -50,6 +50,21 @@ def read_traffic(workload: str):
     return float(m.group(1)) if m else None
 
 
+# FP64 VALU issue peak: one wave64 FP64 instruction per CU per cycle (64 FMA
+# lanes per CU, the 78.6 TFLOP/s vector rate), 256 CUs at 2.4 GHz.
+VALU_PEAK_WAVE_INSTR_PER_S = 256 * 2.4e9
+
+
+def read_valu(workload: str):
+    """VALU wave-instructions per stage launch from the committed rocprofv3
+    --pmc summary (SQ_INSTS_VALU), or None."""
+    path = os.path.join(ROOT, "profiles", f"{workload}_pmc.md")
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"SQ_INSTS_VALU:\s*([0-9.eE+]+) per launch", open(path).read())
+    return float(m.group(1)) if m else None
+
+
 def cpu_baseline(n: int, scheme: str, seed: int):
     """The CPU oracle (OpenMP restatement of the reference routines) timed on a
     bounded sample of the same workload: whole SSP-RK3 steps of the n^3 grid,
@@ -161,6 +176,14 @@ def main() -> int:
         },
         "cpu_baseline": None,
     }
+    valu = read_valu(workload) if (n == 256 and args.scheme == "Z2") else None
+    if valu:
+        # the ceiling that binds: FP64 VALU issue (the counts are per launch of
+        # this workload, taken from the committed counter profile)
+        rate = valu / launch_s
+        out["valu_roofline"] = {"bound": "fp64 valu issue", "instructions_per_launch": valu,
+                                "achieved": round(rate / 1e9, 1), "peak": round(VALU_PEAK_WAVE_INSTR_PER_S / 1e9, 1),
+                                "unit": "G wave-instr/s", "frac": round(rate / VALU_PEAK_WAVE_INSTR_PER_S, 4)}
     if args.copy_calib:
         # plain device-to-device copies of one field (n^3 * 8 bytes each way),
         # the known-size reference for scaling FETCH_SIZE / WRITE_SIZE
