@@ -16,8 +16,9 @@ directions), the FP32-storage error against FP64 (512^3 to t = 0.5), the
 reference's own viscous Burgers runs with the fused reference step against
 the kernel times the reference published for them, and the z-slab form of
 the run (one 256x256x32 slab, one GPU's share of an 8-way 256^3
-decomposition; 8 slabs on one GPU) against the unsplit grid, and the WENO7
-prototype's scheme at 256^3;
+decomposition; 8 slabs on one GPU) against the unsplit grid, the WENO7
+prototype's scheme at 256^3, and the 256^3 WENO5-JS and WENO5-Z runs to
+t = 0.5 (past shock formation);
 --quick skips them.  Progress goes to stderr.
 """
 from __future__ import annotations
@@ -63,6 +64,26 @@ def read_valu(workload: str):
         return None
     m = re.search(r"SQ_INSTS_VALU:\s*([0-9.eE+]+) per launch", open(path).read())
     return float(m.group(1)) if m else None
+
+
+def run_to_half(scheme: str, n: int = 256, seed: int = 3) -> dict:
+    """The 256^3 periodic run from the seeded Fourier field to t = 0.5 (past
+    shock formation), one hipGraph replay per step; steps after t = 0.5 are
+    no-ops on the device, so the loop asks for a few more than needed."""
+    from multigpu_advectiondiffusion_amd import hip, solvers
+
+    s = solvers.PeriodicBurgers(n, scheme=scheme, cfl=0.4)
+    s.set_initial(solvers.ic_fourier(n, seed=seed), t0=0.0, tend=0.5)
+    s.steps(1)  # capture
+    s.set_initial(solvers.ic_fourier(n, seed=seed), t0=0.0, tend=0.5)
+    e0, e1 = hip.Event(), hip.Event()
+    hip.synchronize()
+    e0.record(s.stream)
+    st = s.run(batch=64)
+    e1.record(s.stream)
+    e1.synchronize()
+    wall = e0.elapsed_ms(e1) / 1e3
+    return {"steps": st.steps, "t": st.t, "wall_s": round(wall, 4), "gpts": round(n ** 3 * st.steps / wall / 1e9, 3)}
 
 
 def cpu_baseline(n: int, scheme: str, seed: int):
@@ -219,6 +240,9 @@ def main() -> int:
         f = studies.fp32(n=512)
         out["fp32_storage_512"] = {k: f[k] for k in ("l1", "linf", "steps_f64", "gpts_f64", "gpts_f32")}
         print("fp32 study done", file=sys.stderr, flush=True)
+        for sch in ("JS", "Z2"):
+            out[f"burgers_{sch.lower()}_256_to_t0.5"] = run_to_half(sch)
+        print("runs to t = 0.5 done", file=sys.stderr, flush=True)
         sl = studies.slab_cases()
         out["z_slabs"] = {k: sl[k] for k in ("one_slab_256x256x32_gpts", "eight_slabs_256_gpts", "unsplit_256_gpts",
                                              "one_slab_vs_unsplit", "eight_slabs_vs_unsplit")}

@@ -99,8 +99,8 @@ namespace {
 using SlabKernel = void (*)(const double *, const double *, double *, GridArgs, unsigned, double, int, Ctrl *);
 
 template <int WENO, int P>
-bool pick_slab(unsigned nx, unsigned ny, SlabKernel &fn, int &tx, int &ty) {
-    if (nx % 32 == 0 && ny % 16 == 0) {
+bool pick_slab(unsigned nx, unsigned ny, int tile_pref, SlabKernel &fn, int &tx, int &ty) {
+    if (tile_pref != 16 && nx % 32 == 0 && ny % 16 == 0) {
         fn = k_burgers_slab_stage<double, WENO, P, 32, 16>;
         tx = 32;
         ty = 16;
@@ -115,10 +115,11 @@ bool pick_slab(unsigned nx, unsigned ny, SlabKernel &fn, int &tx, int &ty) {
     return false;
 }
 
-bool pick_slab_kernel(const mgad_scheme &sc, unsigned nx, unsigned ny, SlabKernel &fn, int &tx, int &ty) {
-    if (sc.weno == MGAD_WENO_Z && sc.p == 1) return pick_slab<kWenoZ, 1>(nx, ny, fn, tx, ty);
-    if (sc.weno == MGAD_WENO_Z && sc.p == 2) return pick_slab<kWenoZ, 2>(nx, ny, fn, tx, ty);
-    if (sc.weno == MGAD_WENO_JS) return pick_slab<kWenoJS, 2>(nx, ny, fn, tx, ty);
+bool pick_slab_kernel(const mgad_scheme &sc, unsigned nx, unsigned ny, int tile_pref, SlabKernel &fn, int &tx,
+                      int &ty) {
+    if (sc.weno == MGAD_WENO_Z && sc.p == 1) return pick_slab<kWenoZ, 1>(nx, ny, tile_pref, fn, tx, ty);
+    if (sc.weno == MGAD_WENO_Z && sc.p == 2) return pick_slab<kWenoZ, 2>(nx, ny, tile_pref, fn, tx, ty);
+    if (sc.weno == MGAD_WENO_JS) return pick_slab<kWenoJS, 2>(nx, ny, tile_pref, fn, tx, ty);
     return false;
 }
 
@@ -138,13 +139,19 @@ int exchange(hipStream_t s, unsigned nx, unsigned ny, unsigned nzs, unsigned nsl
 
 int enqueue_slab_step(hipStream_t s, const mgad_scheme &sc, unsigned nx, unsigned ny, unsigned nzs, unsigned nslab,
                       double dx, double dy, double dz, double *u, double *w1, double *w2, Ctrl *ctrl) {
+    // the launch table of the unsplit grid of the same size (environment
+    // overrides, mgad_burgers_set_launch / autotune entries), chunks capped at
+    // one slab
+    const LaunchCfg cfg = launch_cfg(MGAD_F64, sc, nx, ny, nzs * nslab);
     SlabKernel fn = nullptr;
     int tx = 0, ty = 0;
-    if (!pick_slab_kernel(sc, nx, ny, fn, tx, ty)) {
+    if (!pick_slab_kernel(sc, nx, ny, cfg.tile, fn, tx, ty)) {
         set_error(MGAD_ERR_UNSUPPORTED);
         return MGAD_ERR_UNSUPPORTED;
     }
-    const int kc = default_kchunk(static_cast<int>(nx), static_cast<int>(ny), static_cast<int>(nzs * nslab), tx, ty);
+    const int kc = cfg.kchunk > 0 ? cfg.kchunk
+                                  : default_kchunk(static_cast<int>(nx), static_cast<int>(ny),
+                                                   static_cast<int>(nzs * nslab), tx, ty);
     const int kchunk = kc < static_cast<int>(nzs) ? kc : static_cast<int>(nzs);
     const GridArgs g = make_grid(nx, ny, nzs, dx, dy, dz, kchunk);
     const unsigned per = (nx / tx) * (ny / ty) * ((nzs + kchunk - 1) / kchunk);
