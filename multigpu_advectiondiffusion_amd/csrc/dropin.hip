@@ -111,6 +111,54 @@ __global__ __launch_bounds__(256) void k_adv_sweep(const double *__restrict__ u,
     }
 }
 
+// The x sweep with lanes along x (coalesced): a block takes 63 consecutive
+// nodes i0..i0+62 of 4 rows, one row per wave.  The split fluxes of the
+// nodes, with a 3-node halo, go through LDS; lane q computes the face below
+// node i0+q (0 at the line start, the reference's fu_old = 0), so every
+// lane computes one face and Lu[i] = -(F_{i+1/2} - F_{i-1/2})/d on nodes
+// 2..nx-4 of rows 3..ny-4 and planes b0..b0+nb-1 (Compute_dF, SingleGPU/
+// Burgers3d_WENO5/kernels.cu:197-260; MultiGPU/Burgers3d_Baseline/
+// Kernels.cu:225-293).
+constexpr int kXRowNodes = 63;
+
+template <int WENO, int P, bool LOCAL>
+__global__ __launch_bounds__(256) void k_adv_x_rows(const double *__restrict__ u, double *__restrict__ Lu, int nx,
+                                                    int ny, int px, double d, double eps, double alpha, int b0,
+                                                    int nb) {
+    __shared__ double2 S[4][69];  // (f+, f-) of nodes i0-3 .. i0+65
+    __shared__ double F[4][64];   // faces i0-1/2 .. i0+62+1/2
+    const int lane = threadIdx.x & 63, r = threadIdx.x >> 6;
+    const int i0 = 2 + static_cast<int>(blockIdx.x) * kXRowNodes;
+    const long line = static_cast<long>(blockIdx.y) * 4 + r;
+    const long nlines = static_cast<long>(ny - 6) * nb;
+    const bool live = line < nlines;
+    const long lc = live ? line : nlines - 1;  // rows past the end load a valid row and store nothing
+    const int j = 3 + static_cast<int>(lc % (ny - 6)), b = b0 + static_cast<int>(lc / (ny - 6));
+    const unsigned base = static_cast<unsigned>(px) * j + static_cast<unsigned>(px) * ny * b;
+    auto split = [&](double v) {
+        double fp, fm;
+        if (LOCAL)
+            lf_split_local(v, fp, fm);
+        else
+            lf_split(v, alpha, fp, fm);
+        return make_double2(fp, fm);
+    };
+    // node i0-3 is -1 on the first tile: its slot feeds only the face below
+    // node 2, which is 0, so any valid node will do; nodes past the end clamp
+    S[r][lane] = split(u[base + min(max(i0 - 3 + lane, 0), nx - 1)]);
+    if (lane < 5) S[r][64 + lane] = split(u[base + min(i0 + 61 + lane, nx - 1)]);
+    __syncthreads();
+    {
+        const double2 a0 = S[r][lane], a1 = S[r][lane + 1], a2 = S[r][lane + 2], a3 = S[r][lane + 3],
+                      a4 = S[r][lane + 4], a5 = S[r][lane + 5];
+        const double f = weno_face<WENO, P>(a0.x, a1.x, a2.x, a3.x, a4.x, a1.y, a2.y, a3.y, a4.y, a5.y, eps);
+        F[r][lane] = (lane == 0 && i0 == 2) ? 0.0 : f;
+    }
+    __syncthreads();
+    const int i = i0 + lane;
+    if (live && lane < kXRowNodes && i < nx - 3) Lu[base + i] = -(F[r][lane + 1] - F[r][lane]) / d;
+}
+
 // Compute_Laplace_Async arithmetic, kernels.cu:474-476: Lu += 13-point operator.
 __global__ __launch_bounds__(256) void k_diff13_b5(const double *__restrict__ u,
                                                    double *__restrict__ Lu, int nx, int ny, int nz,
@@ -180,6 +228,18 @@ namespace {
 template <int AXIS, int WENO, int P>
 void launch_sweep_t(hipStream_t s, unsigned nx, unsigned ny, unsigned nz, unsigned px, double d,
                     double *u, double *Lu, const AdvScheme &sc, SweepRange r) {
+    if (AXIS == 0 && r.l0 == 2 && r.l1 == static_cast<int>(nx) - 3 && r.zero_first) {
+        // x lines of the reference sweeps: lanes along x
+        const dim3 grid((nx - 5 + kXRowNodes - 1) / kXRowNodes,
+                        static_cast<unsigned>((static_cast<long>(ny - 6) * r.nb + 3) / 4));
+        if (sc.lf == MGAD_LF_LOCAL)
+            hipLaunchKernelGGL((k_adv_x_rows<WENO, P, true>), grid, dim3(256), 0, s, u, Lu, (int)nx, (int)ny, (int)px,
+                               d, sc.eps, sc.alpha, r.b0, r.nb);
+        else
+            hipLaunchKernelGGL((k_adv_x_rows<WENO, P, false>), grid, dim3(256), 0, s, u, Lu, (int)nx, (int)ny,
+                               (int)px, d, sc.eps, sc.alpha, r.b0, r.nb);
+        return;
+    }
     const long na = (AXIS == 0 ? ny : nx) - 6;
     const int nchunks = (r.l1 - r.l0 + kChunk - 1) / kChunk;
     const long total = na * r.nb * nchunks;
