@@ -306,6 +306,10 @@ void lu3d(hipStream_t s, unsigned nx, unsigned ny, unsigned nz, unsigned px, dou
         set_error(MGAD_ERR_INVALID_ARG);
         return;
     }
+    if (1ull * px * ny < (1ull << 29)) {  // planes addressable with 32-bit byte offsets
+        heat13_lu_launch(s, nx, ny, nz, px, u, Lu);
+        return;
+    }
     const DiffCoef c = diff_coef();
     const dim3 grid((nx - 4 + 63) / 64, (ny - 4 + 3) / 4, nz - 4);
     hipLaunchKernelGGL(k_lu3d_13, grid, dim3(256), 0, s, u, Lu, (int)nx, (int)ny, (int)nz, (int)px,

@@ -59,7 +59,8 @@ struct HaloSlot {
 // node is (i0, j0): waves 0 and 1 the R rows below and above (R entries each),
 // waves 2 and 3 the R columns left and right of every row.
 template <int R, int TY>
-__device__ __forceinline__ void heat_halo(HaloSlot (&h)[R], int w, int lane, int i0, int j0, int nx, int ny) {
+__device__ __forceinline__ void heat_halo(HaloSlot (&h)[R], int w, int lane, int i0, int j0, int nx, int ny,
+                                          int px) {
     constexpr int W = kHeatTX + 2 * R;
     static_assert(2 * R * TY <= 128, "x halo exceeds waves 2 and 3");
     if (w < 2) {
@@ -68,7 +69,7 @@ __device__ __forceinline__ void heat_halo(HaloSlot (&h)[R], int w, int lane, int
             const int jj = w == 0 ? max(j0 - R + e, 0) : min(j0 + TY + e, ny - 1);
             h[e].row = w == 0 ? e : TY + R + e;
             h[e].col = lane + R;
-            h[e].off = 8u * static_cast<unsigned>(min(i0 + lane, nx - 1) + nx * jj);
+            h[e].off = 8u * static_cast<unsigned>(min(i0 + lane, nx - 1) + px * jj);
         }
     } else {
         const int t = (w - 2) * 64 + lane;
@@ -77,7 +78,7 @@ __device__ __forceinline__ void heat_halo(HaloSlot (&h)[R], int w, int lane, int
             const int ii = c < R ? max(i0 - R + c, 0) : min(i0 + kHeatTX + c - R, nx - 1);
             h[0].row = rr + R;
             h[0].col = c < R ? c : W - 2 * R + c;
-            h[0].off = 8u * static_cast<unsigned>(ii + nx * min(j0 + rr, ny - 1));
+            h[0].off = 8u * static_cast<unsigned>(ii + px * min(j0 + rr, ny - 1));
         }
     }
 }
@@ -111,7 +112,7 @@ __global__ __launch_bounds__(256) void k_heat7_fe(const double *__restrict__ u, 
         keep[r] = i == 0 || j == 0 || i == nx - 1 || j == ny - 1;
     }
     HaloSlot hs[1];
-    heat_halo<1, TY>(hs, w, lane, i0, j0, nx, ny);
+    heat_halo<1, TY>(hs, w, lane, i0, j0, nx, ny, nx);
     const bool hal = hs[0].row >= 0;
     const unsigned ho = hal ? hs[0].off : lo[0];  // lanes without a halo node repeat their own
     const double ckx = 12 * kx, cky = 12 * ky, ckz = 12 * kz;
@@ -171,10 +172,10 @@ __global__ __launch_bounds__(256) void k_heat7_fe(const double *__restrict__ u, 
 // the interior 2..n-3 of every axis, Lu kept in a register.  Loads run D
 // planes ahead of their first use: planes of the own columns sit in an
 // 8-slot register ring (k-2..k+2 in use), the halo and un in 4-slot rings.
-template <int TY, int D>
+template <int TY, int D, bool LU = false>
 __global__ __launch_bounds__(256) void k_heat13_rk(const double *__restrict__ uin, const double *un, double *uout,
-                                                   int nx, int ny, int nz, int ntx, int nty, int kchunk, double kx,
-                                                   double ky, double kz, double dt, int stage) {
+                                                   int nx, int ny, int nz, int px, int ntx, int nty, int kchunk,
+                                                   double kx, double ky, double kz, double dt, int stage) {
     static_assert(D >= 1 && D <= 3, "prefetch distance");
     constexpr int RPT = TY / 4, W = kHeatTX + 4;
     __shared__ double sp[2][TY + 4][W];
@@ -183,17 +184,17 @@ __global__ __launch_bounds__(256) void k_heat13_rk(const double *__restrict__ ui
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int i0 = 2 + tix * kHeatTX, j0 = 2 + tiy * TY, z0 = 2 + tiz * kchunk, z1 = min(z0 + kchunk, nz - 2);
     const int i = i0 + lane;
-    const size_t xy = static_cast<size_t>(nx) * ny;
+    const size_t xy = static_cast<size_t>(px) * ny;
     const unsigned pb = static_cast<unsigned>(xy * 8);
     unsigned lo[RPT], so[RPT];  // load and store offsets; so = pb drops the store
 #pragma unroll
     for (int r = 0; r < RPT; r++) {
         const int j = j0 + w + 4 * r;
-        lo[r] = 8u * static_cast<unsigned>(min(i, nx - 1) + nx * min(j, ny - 1));
+        lo[r] = 8u * static_cast<unsigned>(min(i, nx - 1) + px * min(j, ny - 1));
         so[r] = i < nx - 2 && j < ny - 2 ? lo[r] : pb;
     }
     HaloSlot hs[2];
-    heat_halo<2, TY>(hs, w, lane, i0, j0, nx, ny);
+    heat_halo<2, TY>(hs, w, lane, i0, j0, nx, ny, px);
     const unsigned ho0 = hs[0].row >= 0 ? hs[0].off : lo[0], ho1 = hs[1].row >= 0 ? hs[1].off : lo[0];
     // un is read only by stages 2 and 3; stage 1 repeats uin loads instead
     const double *ur = stage == 1 ? uin : un;
@@ -212,8 +213,10 @@ __global__ __launch_bounds__(256) void k_heat13_rk(const double *__restrict__ ui
         const PlaneBuf ph(uin + xy * min(z0 + t, zh), pb), po(ur + xy * min(z0 + t, zh), pb);
         h0[t] = ph.load(ho0);
         h1[t] = ph.load(ho1);
+        if (!LU) {
 #pragma unroll
-        for (int r = 0; r < RPT; r++) o[r][t] = po.load(lo[r]);
+            for (int r = 0; r < RPT; r++) o[r][t] = po.load(lo[r]);
+        }
     }
     int kb = z0;
     auto plane = [&](auto S) {
@@ -225,8 +228,10 @@ __global__ __launch_bounds__(256) void k_heat13_rk(const double *__restrict__ ui
         for (int r = 0; r < RPT; r++) q[r][(s + 4 + D) & 7] = pn.load(lo[r]);
         h0[(s + D) & 3] = ph.load(ho0);
         h1[(s + D) & 3] = ph.load(ho1);
+        if (!LU) {
 #pragma unroll
-        for (int r = 0; r < RPT; r++) o[r][(s + D) & 3] = po.load(lo[r]);
+            for (int r = 0; r < RPT; r++) o[r][(s + D) & 3] = po.load(lo[r]);
+        }
         double(*P)[W] = sp[s & 1];
 #pragma unroll
         for (int r = 0; r < RPT; r++) P[w + 4 * r + 2][lane + 2] = q[r][(s + 2) & 7];
@@ -242,8 +247,12 @@ __global__ __launch_bounds__(256) void k_heat13_rk(const double *__restrict__ ui
             const double Y = d4(P[row + 2][x], P[row + 1][x], c, P[row - 1][x], P[row - 2][x]);
             const double Z = d4(q[r][(s + 4) & 7], q[r][(s + 3) & 7], c, q[r][(s + 1) & 7], q[r][s & 7]);
             const double Lu = (kx * X + ky * Y) + kz * Z;
-            const double uo = stage == 1 ? c : o[r][s & 3];
-            pv.store(so[r], rk_stage(stage, dt, c, uo, Lu));
+            if (LU) {
+                pv.store(so[r], Lu);
+            } else {
+                const double uo = stage == 1 ? c : o[r][s & 3];
+                pv.store(so[r], rk_stage(stage, dt, c, uo, Lu));
+            }
         }
     };
     for (; kb < z1; kb += 8) {
@@ -329,11 +338,11 @@ int enqueue_heat13(hipStream_t s, unsigned nx, unsigned ny, unsigned nz, DiffCoe
                    double *w1, double *w2) {
     const HeatGrid g = heat13_grid(nx - 4, ny - 4, nz - 4);
     const auto kern = k_heat13_rk<kHeat13TY, kHeat13D>;
-    hipLaunchKernelGGL(kern, dim3(g.blocks), dim3(256), 0, s, u, u, w1, (int)nx, (int)ny, (int)nz,
+    hipLaunchKernelGGL(kern, dim3(g.blocks), dim3(256), 0, s, u, u, w1, (int)nx, (int)ny, (int)nz, (int)nx,
                        g.ntx, g.nty, g.kc, c.kx, c.ky, c.kz, dt, 1);
-    hipLaunchKernelGGL(kern, dim3(g.blocks), dim3(256), 0, s, w1, u, w2, (int)nx, (int)ny, (int)nz,
+    hipLaunchKernelGGL(kern, dim3(g.blocks), dim3(256), 0, s, w1, u, w2, (int)nx, (int)ny, (int)nz, (int)nx,
                        g.ntx, g.nty, g.kc, c.kx, c.ky, c.kz, dt, 2);
-    hipLaunchKernelGGL(kern, dim3(g.blocks), dim3(256), 0, s, w2, u, u, (int)nx, (int)ny, (int)nz,
+    hipLaunchKernelGGL(kern, dim3(g.blocks), dim3(256), 0, s, w2, u, u, (int)nx, (int)ny, (int)nz, (int)nx,
                        g.ntx, g.nty, g.kc, c.kx, c.ky, c.kz, dt, 3);
     return check_launch();
 }
@@ -354,6 +363,23 @@ int enqueue_heat(hipStream_t s, unsigned nx, unsigned ny, unsigned nz, double kx
                                         hipMemcpyDeviceToDevice, s));
     return MGAD_OK;
 }
+
+}  // namespace
+
+// Compute_Laplace3d (Diffusion3d_baselineCode/kernels.cu:94-97) into Lu on
+// the interior 2..n-3 with the heat stage kernel's LDS z-march (Call_Lu3d):
+// row pitch px, planes of px*ny < 2^29 values.
+int heat13_lu_launch(hipStream_t s, unsigned nx, unsigned ny, unsigned nz, unsigned px, const double *u,
+                     double *Lu) {
+    const HeatGrid g = heat13_grid(nx - 4, ny - 4, nz - 4);
+    const DiffCoef c = diff_coef();
+    const auto kern = k_heat13_rk<kHeat13TY, kHeat13D, true>;
+    hipLaunchKernelGGL(kern, dim3(g.blocks), dim3(256), 0, s, u, u, Lu, (int)nx, (int)ny, (int)nz, (int)px, g.ntx,
+                       g.nty, g.kc, c.kx, c.ky, c.kz, 0.0, 1);
+    return check_launch();
+}
+
+namespace {
 
 struct HeatKey {
     hipStream_t s;

@@ -24,6 +24,9 @@ int check_launch();                    // hipGetLastError after a launch
 // Diffusion coefficients of CopyToConstantMemory, passed to kernels by value.
 struct DiffCoef { double kx, ky, kz; };
 DiffCoef diff_coef();
+// Call_Lu3d's operator on the heat stage kernel (heat.hip).
+int heat13_lu_launch(hipStream_t s, unsigned nx, unsigned ny, unsigned nz, unsigned px, const double *u,
+                     double *Lu);
 
 // ---------------------------------------------------------------------------
 // Device: WENO5 arithmetic.
