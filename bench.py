@@ -8,17 +8,21 @@ JSON line (the contract the harness reads).
     python bench.py [--steps K] [--warmup W] [--n 256] [--scheme Z2|JS]
                     [--no-cpu] [--quick] [--gpus 1]
 
-After the timed region (and the CPU baseline) it also runs the three other
-measurements the metric names, reported as extra fields: the batched
-uneven-size speedup over one-by-one launches (64 problems, 200 steps), the
-adjoint cost ratio and finite-difference agreement (128^3, 400 steps / 16
-directions), the FP32-storage error against FP64 (512^3 to t = 0.5), the
-reference's own viscous Burgers runs with the fused reference step against
-the kernel times the reference published for them, and the z-slab form of
-the run (one 256x256x32 slab, one GPU's share of an 8-way 256^3
-decomposition; 8 slabs on one GPU) against the unsplit grid, the WENO7
-prototype's scheme at 256^3, and the 256^3 WENO5-JS and WENO5-Z runs to
-t = 0.5 (past shock formation);
+It also reports the FP64 VALU issue roofline of the same launches (the
+ceiling that binds; instruction counts from the committed counter profile).
+After the timed region (and the CPU baseline) it runs the other measurements
+the metric names, plus the reference's own runs, as extra fields:
+  - batched: 64 uneven problems, one persistent launch per stage, against
+    one-by-one launches (200 steps);
+  - adjoint: cost ratio and finite-difference agreement (128^3, 400 steps,
+    16 directions);
+  - fp32_storage_512: FP32-storage error against FP64 (512^3 to t = 0.5);
+  - reference_cases: the reference's own viscous Burgers runs with the fused
+    reference step, against the kernel times it published;
+  - burgers_js/z2_256_to_t0.5: the 256^3 runs to t = 0.5 (past shocks);
+  - z_slabs: one 256x256x32 slab (one GPU's share of an 8-way 256^3
+    decomposition) and 8 slabs on one GPU, against the unsplit grid;
+  - weno7_256: the WENO7 prototype's scheme at 256^3.
 --quick skips them.  Progress goes to stderr.
 """
 from __future__ import annotations
