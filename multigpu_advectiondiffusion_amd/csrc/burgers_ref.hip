@@ -27,6 +27,10 @@ namespace mgad {
 struct RefArgs {
     int nx, ny, nz, kchunk, ntx, nty;
     double dx, dy, dz, rdx, rdy, rdz;
+    // max|u| slots: [0] u^n, [1] u^(1), [2] u^(2) (interior values written by
+    // the stages), [3] all of u at the call's start (covers the ghost layers,
+    // which no stage writes)
+    unsigned long long *amax;
 };
 
 template <int TX, int TY>
@@ -35,17 +39,20 @@ struct RefSmem {
     double U[TY + 6][TX + 6];   // u of the same nodes (viscous term)
     double FX[TY][TX + 1];
     double FY[TY + 1][TX];
+    unsigned long long red[TX * TY / 64];
 };
 
-template <int WENO, int P, bool VISC, int TX, int TY>
-__global__ __launch_bounds__(TX * TY, 4) void k_burgers_ref_stage(const double *__restrict__ uin,
-                                                                  const double *un, double *uout, RefArgs g,
-                                                                  int stage, double dt, double eps,
-                                                                  DiffCoef kc) {
+// One tile x z-chunk of one stage; returns the largest |u_out| bit pattern
+// this thread stored.  FD: fast_division_ok<WENO>(max|u|, eps) holds for the
+// stage's input (the weight divisions then take the shortened sequence).
+template <int WENO, int P, bool VISC, int TX, int TY, bool FD>
+__device__ __forceinline__ unsigned long long ref_stage_tile(const double *__restrict__ uin, const double *un,
+                                                             double *uout, const RefArgs &g, int stage, double dt,
+                                                             double eps, const DiffCoef &kc, RefSmem<TX, TY> &sm) {
     constexpr int NT = TX * TY;
     constexpr int NW = NT / 64;
     static_assert(NT % 64 == 0 && 6 * TX + 6 * TY <= NT && TX + TY <= 64, "tile shape");
-    __shared__ RefSmem<TX, TY> sm;
+    unsigned long long vmax = 0ull;
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
     const int bx = tile % g.ntx, by = (tile / g.ntx) % g.nty, bz = tile / (g.ntx * g.nty);
@@ -90,8 +97,8 @@ __global__ __launch_bounds__(TX * TY, 4) void k_burgers_ref_stage(const double *
         sp[q] = sterm(qp[q + 1], qp[q + 2], qp[q + 3]);
         sn[q] = sterm(qm[q + 1], qm[q + 2], qm[q + 3]);
     }
-    double fz_prev = weno_face_s<WENO, P>(qp[0], qp[1], qp[2], qp[3], qp[4], qm[1], qm[2], qm[3], qm[4], qm[5],
-                                          sp0, sp[0], sp[1], sn[0], sn[1], sn[2], eps);
+    double fz_prev = weno_face_s<WENO, P, FD>(qp[0], qp[1], qp[2], qp[3], qp[4], qm[1], qm[2], qm[3], qm[4], qm[5],
+                                              sp0, sp[0], sp[1], sn[0], sn[1], sn[2], eps);
     double own_next = uin[col + plane * static_cast<unsigned>(z0 + 3)];
     double halo_next = has_halo ? uin[hcol + plane * static_cast<unsigned>(z0)] : 0.0;
     double un_next = stage == 1 ? 0.0 : un[col + plane * static_cast<unsigned>(z0)];
@@ -125,19 +132,19 @@ __global__ __launch_bounds__(TX * TY, 4) void k_burgers_ref_stage(const double *
             X = d4(sm.U[ty + 3][tx + 1], sm.U[ty + 3][tx + 2], c, sm.U[ty + 3][tx + 4], sm.U[ty + 3][tx + 5]);
             Y = d4(sm.U[ty + 1][tx + 3], sm.U[ty + 2][tx + 3], c, sm.U[ty + 4][tx + 3], sm.U[ty + 5][tx + 3]);
         }
-        const double fz = weno_face_s<WENO, P>(qp[1], qp[2], qp[3], qp[4], qp[5], qm[2], qm[3], qm[4], qm[5], qm[6],
-                                               sp[0], sp[1], sp[2], sn[1], sn[2], sn[3], eps);
+        const double fz = weno_face_s<WENO, P, FD>(qp[1], qp[2], qp[3], qp[4], qp[5], qm[2], qm[3], qm[4], qm[5],
+                                                   qm[6], sp[0], sp[1], sp[2], sn[1], sn[2], sn[3], eps);
         {
             const double2 a0 = sm.P[ty + 3][tx + 1], a1 = sm.P[ty + 3][tx + 2], a2 = sm.P[ty + 3][tx + 3],
                           a3 = sm.P[ty + 3][tx + 4], a4 = sm.P[ty + 3][tx + 5], a5 = sm.P[ty + 3][tx + 6];
             sm.FX[ty][tx + 1] =
-                weno_face2<WENO, P>(a0.x, a1.x, a2.x, a3.x, a4.x, a1.y, a2.y, a3.y, a4.y, a5.y, eps);
+                weno_face2<WENO, P, FD>(a0.x, a1.x, a2.x, a3.x, a4.x, a1.y, a2.y, a3.y, a4.y, a5.y, eps);
         }
         {
             const double2 a0 = sm.P[ty + 1][tx + 3], a1 = sm.P[ty + 2][tx + 3], a2 = sm.P[ty + 3][tx + 3],
                           a3 = sm.P[ty + 4][tx + 3], a4 = sm.P[ty + 5][tx + 3], a5 = sm.P[ty + 6][tx + 3];
             sm.FY[ty + 1][tx] =
-                weno_face2<WENO, P>(a0.x, a1.x, a2.x, a3.x, a4.x, a1.y, a2.y, a3.y, a4.y, a5.y, eps);
+                weno_face2<WENO, P, FD>(a0.x, a1.x, a2.x, a3.x, a4.x, a1.y, a2.y, a3.y, a4.y, a5.y, eps);
         }
         if (wave == k % NW && lane < TY + TX) {
             double2 a0, a1, a2, a3, a4, a5;
@@ -150,7 +157,8 @@ __global__ __launch_bounds__(TX * TY, 4) void k_burgers_ref_stage(const double *
                 a0 = sm.P[0][c]; a1 = sm.P[1][c]; a2 = sm.P[2][c];
                 a3 = sm.P[3][c]; a4 = sm.P[4][c]; a5 = sm.P[5][c];
             }
-            const double f = weno_face2<WENO, P>(a0.x, a1.x, a2.x, a3.x, a4.x, a1.y, a2.y, a3.y, a4.y, a5.y, eps);
+            const double f =
+                weno_face2<WENO, P, FD>(a0.x, a1.x, a2.x, a3.x, a4.x, a1.y, a2.y, a3.y, a4.y, a5.y, eps);
             if (lane < TY)
                 sm.FX[lane][0] = f;
             else
@@ -173,7 +181,11 @@ __global__ __launch_bounds__(TX * TY, 4) void k_burgers_ref_stage(const double *
             unew = 0.75 * uo + 0.25 * (uq[2] + dt * Lu);
         else
             unew = div_by(p2fma(2.0, uq[2] + dt * Lu, uo), 3.0, 1.0 / 3.0);  // (uo + 2(u + dt Lu)) / 3
-        if (active) uout[col + plane * static_cast<unsigned>(k)] = unew;
+        if (active) {
+            uout[col + plane * static_cast<unsigned>(k)] = unew;
+            const unsigned long long bb = abs_bits(unew);
+            vmax = bb > vmax ? bb : vmax;
+        }
 
         fz_prev = fz;
 #pragma unroll
@@ -189,7 +201,64 @@ __global__ __launch_bounds__(TX * TY, 4) void k_burgers_ref_stage(const double *
             sn[q] = sn[q + 1];
         }
     }
+    return vmax;
 }
+
+// One stage over all tiles: the stage's max|u| bound (its input's slot and
+// the ghost floor) picks the division sequence for every block alike; each
+// block folds its stored |u_out| into the output slot, and the lead clears
+// the slot the next-but-one stage fills.
+template <int WENO, int P, bool VISC, int TX, int TY>
+__global__ __launch_bounds__(TX * TY, 4) void k_burgers_ref_stage(const double *__restrict__ uin,
+                                                                  const double *un, double *uout, RefArgs g,
+                                                                  int stage, double dt, double eps,
+                                                                  DiffCoef kc) {
+    constexpr int NW = TX * TY / 64;
+    __shared__ RefSmem<TX, TY> sm;
+    const unsigned long long a_in = g.amax[stage - 1], a_gh = g.amax[3];
+    const double amax = bits_double(a_in > a_gh ? a_in : a_gh);
+    if (blockIdx.x == 0 && threadIdx.x == 0) g.amax[(stage + 1) % 3] = 0ull;
+    unsigned long long vmax;
+    if (fast_division_ok<WENO>(amax, eps))
+        vmax = ref_stage_tile<WENO, P, VISC, TX, TY, true>(uin, un, uout, g, stage, dt, eps, kc, sm);
+    else
+        vmax = ref_stage_tile<WENO, P, VISC, TX, TY, false>(uin, un, uout, g, stage, dt, eps, kc, sm);
+    vmax = wave_max_u64(vmax);
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) sm.red[wave] = vmax;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long m = sm.red[0];
+#pragma unroll
+        for (int w = 1; w < NW; w++) m = sm.red[w] > m ? sm.red[w] : m;
+        atomicMax(&g.amax[stage % 3], m);
+    }
+}
+
+// Slots for a call: [0] = [3] = max|u| over all nodes, [1] = [2] = 0.
+__global__ void k_ref_amax_clear(unsigned long long *amax) {
+    amax[0] = amax[1] = amax[2] = amax[3] = 0ull;
+}
+
+__global__ __launch_bounds__(256) void k_ref_amax_init(const double *__restrict__ u, size_t n,
+                                                       unsigned long long *amax) {
+    __shared__ unsigned long long red[4];
+    unsigned long long m = 0ull;
+    for (size_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += 256ull * gridDim.x) {
+        const unsigned long long b = abs_bits(u[i]);
+        m = b > m ? b : m;
+    }
+    m = wave_max_u64(m);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        m = red[0];
+        for (int w = 1; w < 4; w++) m = red[w] > m ? red[w] : m;
+        atomicMax(&amax[0], m);
+        atomicMax(&amax[3], m);
+    }
+}
+
 
 namespace {
 
@@ -249,6 +318,18 @@ int enqueue_ref_step(hipStream_t s, const RefLaunch &L, double dt, double eps, D
     return check_launch();
 }
 
+// The max|u| slots of reference-step runs on a stream (4 per stream, kept
+// for the process: graphs captured on the stream point at them).
+unsigned long long *ref_amax_slots(hipStream_t s) {
+    static std::map<hipStream_t, unsigned long long *> slots;
+    auto it = slots.find(s);
+    if (it != slots.end()) return it->second;
+    unsigned long long *p = nullptr;
+    if (check_hip(hipMalloc(&p, 4 * sizeof(unsigned long long)))) return nullptr;
+    slots[s] = p;
+    return p;
+}
+
 using RefGraphKey = std::tuple<hipStream_t, int, int, double, int, unsigned, unsigned, unsigned, double, double,
                                double, double, double, double, double, double *, double *, double *>;
 
@@ -277,7 +358,14 @@ extern "C" int mgad_burgers_ref_steps(hipStream_t stream, const mgad_scheme *sch
     int st = check_hip(hipMemcpyAsync(w1, u, bytes, hipMemcpyDeviceToDevice, stream));
     if (!st) st = check_hip(hipMemcpyAsync(w2, u, bytes, hipMemcpyDeviceToDevice, stream));
     if (st || nsteps == 0) return st;
-    const RefLaunch L = ref_launch(*scheme, viscous != 0, nx, ny, nz, dx, dy, dz);
+    RefLaunch L = ref_launch(*scheme, viscous != 0, nx, ny, nz, dx, dy, dz);
+    L.g.amax = ref_amax_slots(stream);
+    if (!L.g.amax) return MGAD_ERR_HIP;
+    hipLaunchKernelGGL(k_ref_amax_clear, dim3(1), dim3(1), 0, stream, L.g.amax);
+    const size_t n = 1ull * nx * ny * nz;
+    hipLaunchKernelGGL(k_ref_amax_init, dim3(static_cast<unsigned>(std::min<size_t>((n + 255) / 256, 2048))),
+                       dim3(256), 0, stream, u, n, L.g.amax);
+    if ((st = check_launch())) return st;
     const DiffCoef kc = viscous ? diff_coef() : DiffCoef{0.0, 0.0, 0.0};
     if (!use_graph || stream == nullptr) {
         for (int s = 0; s < nsteps && !st; s++) st = enqueue_ref_step(stream, L, dt, scheme->eps, kc, u, w1, w2);
