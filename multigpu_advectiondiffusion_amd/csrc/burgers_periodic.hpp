@@ -101,12 +101,16 @@ struct TileSmem {
 // stage's max|u| slot.  FD: fast_division_ok<WENO>(alpha, eps) holds.
 // ZG: the field is one z slab of nz planes stored between 3 ghost planes
 // below and 3 above (plane k at storage plane k + 3, read for k in -3..nz+2,
-// written for k in 0..nz-1); otherwise z is periodic over nz planes.
+// written for k in 0..nz-1); otherwise z is periodic over nz planes.  With
+// ZG, planes 0..2 are also stored into the top ghost planes of the slab set
+// at gh_lo and planes nz-3..nz-1 into the bottom ghost planes at gh_hi (the
+// neighbours' copies of them, so no separate exchange is needed).
 template <class T, int WENO, int P, int TX, int TY, bool FD, bool ZG = false>
 __device__ __forceinline__ void stage_tile(const T *__restrict__ uin, const T *un, T *uout,
                                            const GridArgs &g, int stage, unsigned tile, double dt,
                                            double alpha, double eps, TileSmem<TX, TY> &sm,
-                                           unsigned long long *amax_slot) {
+                                           unsigned long long *amax_slot, T *gh_lo = nullptr,
+                                           T *gh_hi = nullptr) {
     constexpr int NT = TX * TY;
     constexpr int NW = NT / 64;
     static_assert(NT % 64 == 0, "whole waves");
@@ -247,6 +251,10 @@ __device__ __forceinline__ void stage_tile(const T *__restrict__ uin, const T *u
             unew = div_by(p2fma(2.0, qu[0] + dt * Lu, uo), 3.0, 1.0 / 3.0);  // (uo + 2(..)) / 3
         const T stored = static_cast<T>(unew);
         out_ptr[plane * zi(k)] = stored;
+        if (ZG) {
+            if (k < 3) gh_lo[col + plane * static_cast<unsigned>(nz + 3 + k)] = stored;
+            if (k >= nz - 3) gh_hi[col + plane * static_cast<unsigned>(k - nz + 3)] = stored;
+        }
         const unsigned long long b = abs_bits(static_cast<double>(stored));
         vmax = b > vmax ? b : vmax;
 

@@ -10,10 +10,13 @@
 // planes of each neighbour.  The x-fastest layout makes every boundary region
 // 3 contiguous planes: a neighbour exchange is a plain copy of
 // 3*nx*ny values per side, with no packing kernel.  Here all slabs live on
-// one device and mgad_slab_exchange copies the planes between them (the
-// periodic wrap included); across devices the same copies are the
-// send/recv of each stage.  Every stage reads its input's ghosts, so the
-// exchange runs after each stage, on u (the reference exchanges the RHS).
+// one device, and each stage writes the boundary planes of its output into
+// the neighbours' ghost planes as it stores them (the periodic wrap
+// included), so the next stage finds its input's ghosts filled;
+// mgad_slab_exchange does the same copy on its own (the initial field).
+// Across devices the same stores go to the neighbour's memory (peer stores)
+// or become the send/recv of each stage.  The exchange is of u after every
+// stage (the reference exchanges the RHS).
 // All slabs of a stage run in one launch and share one control block, so
 // max|u| and dt are global over the slabs.
 #include <hip/hip_runtime.h>
@@ -35,13 +38,16 @@ __global__ __launch_bounds__(TX * TY, 4) void k_burgers_slab_stage(const T *__re
     if (!stage_control(ctrl, stage, g.dx, lead, dt, alpha)) return;
     const unsigned t = xcd_tile(blockIdx.x, gridDim.x), per = gridDim.x / nslab;
     const unsigned slab = t / per, tile = t % per;
-    const size_t off = static_cast<size_t>(slab) * (g.nz + 6) * static_cast<size_t>(g.nx) * g.ny;
+    const size_t stride = static_cast<size_t>(g.nz + 6) * static_cast<size_t>(g.nx) * g.ny;
+    const size_t off = slab * stride;
+    // the neighbours whose ghost planes this slab's boundary planes fill
+    T *lo = uout + ((slab + nslab - 1) % nslab) * stride, *hi = uout + ((slab + 1) % nslab) * stride;
     if (fast_division_ok<WENO>(alpha, eps))
         stage_tile<T, WENO, P, TX, TY, true, true>(uin + off, un + off, uout + off, g, stage, tile, dt, alpha,
-                                                   eps, sm, &ctrl->amax[stage % 3]);
+                                                   eps, sm, &ctrl->amax[stage % 3], lo, hi);
     else
         stage_tile<T, WENO, P, TX, TY, false, true>(uin + off, un + off, uout + off, g, stage, tile, dt, alpha,
-                                                    eps, sm, &ctrl->amax[stage % 3]);
+                                                    eps, sm, &ctrl->amax[stage % 3], lo, hi);
 }
 
 // Ghost planes of every slab from its neighbours: storage planes 0..2 of
@@ -160,8 +166,7 @@ int enqueue_slab_step(hipStream_t s, const mgad_scheme &sc, unsigned nx, unsigne
     double *out[3] = {w1, w2, u};
     for (int stage = 1; stage <= 3; stage++) {
         hipLaunchKernelGGL(fn, grid, block, 0, s, in[stage - 1], u, out[stage - 1], g, nslab, sc.eps, stage, ctrl);
-        int st = check_launch();
-        if (!st) st = exchange(s, nx, ny, nzs, nslab, out[stage - 1]);
+        const int st = check_launch();
         if (st) return st;
     }
     return MGAD_OK;
