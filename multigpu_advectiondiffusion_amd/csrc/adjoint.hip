@@ -9,14 +9,15 @@
 // first node attaining the maximum) and dt = cfl dx / a1, except a last step
 // clipped to T, whose dt = T - sum of the earlier dt's.
 //
-// The VJP of L along one axis is a line march: the thread owning a chunk of
-// nodes walks the faces that touch them, recomputes each face flux and its
-// gradient with respect to its ten split-flux inputs, and scatters
-// Fbar * gradient into a six-node register accumulator; a node is finished
-// (and written, by its owner only) once the face two past it is done, so the
-// result does not depend on thread timing.  Block partial sums of
-// <lam, L> and <lam, dL/dalpha> feed the dt and alpha adjoints through a
-// one-thread kernel per stage, all on the device.
+// Each face is recomputed once in reverse mode, seeded with its flux adjoint
+// Fbar, which gives Fbar times the gradient of the face flux with respect to
+// its ten split-flux inputs.  These terms are summed per node, separately for
+// f+ and f-, in a fixed order (through LDS for the x and y faces of a tile,
+// in register queues along a z march), and the split's chain rule is applied
+// once per node.  Every node is written by its owner only, so the result does
+// not depend on thread timing.  Block partial sums of <lam, L> and
+// <lam, dL/dalpha> give the dt and alpha adjoints, folded by one block of a
+// short launch per stage, all on the device.
 //
 // This file is built with FMA contraction and uses a Newton-refined hardware
 // reciprocal: the gradient is checked against finite differences and the CPU
@@ -31,19 +32,19 @@
 
 namespace mgad {
 
+// 1/x for normal x: the hardware reciprocal (relative error below 2^-24 on
+// gfx950) and one Newton step, which squares it (below 2^-48).
 __device__ __forceinline__ double frcp(double x) {
-    double r = __builtin_amdgcn_rcp(x);
-    double e = __builtin_fma(-x, r, 1.0);
-    r = __builtin_fma(r, e, r);
-    e = __builtin_fma(-x, r, 1.0);
-    return __builtin_fma(r, e, r);
+    const double r = __builtin_amdgcn_rcp(x);
+    return __builtin_fma(r, __builtin_fma(-x, r, 1.0), r);
 }
 
-// Value and gradient of one reconstruction side (reverse mode, unit seed);
-// same formulas as the forward weno_side_s.
-template <int WENO, int P, bool PLUS>
-__device__ __forceinline__ double side_grad(double v0, double v1, double v2, double v3, double v4, double eps,
-                                            double *g) {
+// Value h of one reconstruction side (same formulas as the forward
+// weno_side_s) and, in reverse mode with the seed on h, seed * dh/dv_t for
+// t = 0..4: stored in g[t] (ACC false) or added to it (ACC true).
+template <int WENO, int P, bool PLUS, bool ACC>
+__device__ __forceinline__ double side_vjp(double v0, double v1, double v2, double v3, double v4, double eps,
+                                           double seed, double *g) {
     const double s0 = (v0 - 2.0 * v1) + v2, t0 = (v0 - 4.0 * v1) + 3.0 * v2;
     const double s1 = (v1 - 2.0 * v2) + v3, t1 = v1 - v3;
     const double s2 = (v2 - 2.0 * v3) + v4, t2 = (3.0 * v2 - 4.0 * v3) + v4;
@@ -71,9 +72,10 @@ __device__ __forceinline__ double side_grad(double v0, double v1, double v2, dou
     }
     const double S = a0 + a1 + a2;
     const double rS = frcp(S);
-    const double h = (a0 * c0 + a1 * c1 + a2 * c2) * (rS * (1.0 / 6.0));
+    const double rS6 = rS * (1.0 / 6.0);
+    const double h = (a0 * c0 + a1 * c1 + a2 * c2) * rS6;
     // reverse
-    const double Nb = rS * (1.0 / 6.0), Sb = -h * rS;
+    const double Nb = seed * rS6, Sb = -(h * seed) * rS;
     const double ab0 = Nb * c0 + Sb, ab1 = Nb * c1 + Sb, ab2 = Nb * c2 + Sb;
     const double cb0 = Nb * a0, cb1 = Nb * a1, cb2 = Nb * a2;
     double Bb0, Bb1, Bb2;
@@ -94,19 +96,34 @@ __device__ __forceinline__ double side_grad(double v0, double v1, double v2, dou
     const double s0b = 2.0 * kC1312 * s0 * Bb0, t0b = 2.0 * kC14 * t0 * Bb0;
     const double s1b = 2.0 * kC1312 * s1 * Bb1, t1b = 2.0 * kC14 * t1 * Bb1;
     const double s2b = 2.0 * kC1312 * s2 * Bb2, t2b = 2.0 * kC14 * t2 * Bb2;
-    g[0] = s0b + t0b;
-    g[1] = -2.0 * s0b - 4.0 * t0b + s1b + t1b;
-    g[2] = s0b + 3.0 * t0b - 2.0 * s1b + s2b + 3.0 * t2b;
-    g[3] = s1b - t1b - 2.0 * s2b - 4.0 * t2b;
-    g[4] = s2b + t2b;
+    double e[5];
+    e[0] = s0b + t0b;
+    e[1] = -2.0 * s0b - 4.0 * t0b + s1b + t1b;
+    e[2] = s0b + 3.0 * t0b - 2.0 * s1b + s2b + 3.0 * t2b;
+    e[3] = s1b - t1b - 2.0 * s2b - 4.0 * t2b;
+    e[4] = s2b + t2b;
     if (PLUS) {
-        g[0] += 2.0 * cb0; g[1] += -7.0 * cb0 - cb1; g[2] += 11.0 * cb0 + 5.0 * cb1 + 2.0 * cb2;
-        g[3] += 2.0 * cb1 + 5.0 * cb2; g[4] += -cb2;
+        e[0] += 2.0 * cb0; e[1] += -7.0 * cb0 - cb1; e[2] += 11.0 * cb0 + 5.0 * cb1 + 2.0 * cb2;
+        e[3] += 2.0 * cb1 + 5.0 * cb2; e[4] += -cb2;
     } else {
-        g[0] += -cb0; g[1] += 5.0 * cb0 + 2.0 * cb1; g[2] += 2.0 * cb0 + 5.0 * cb1 + 11.0 * cb2;
-        g[3] += -cb1 - 7.0 * cb2; g[4] += 2.0 * cb2;
+        e[0] += -cb0; e[1] += 5.0 * cb0 + 2.0 * cb1; e[2] += 2.0 * cb0 + 5.0 * cb1 + 11.0 * cb2;
+        e[3] += -cb1 - 7.0 * cb2; e[4] += 2.0 * cb2;
     }
+#pragma unroll
+    for (int t = 0; t < 5; t++) g[t] = ACC ? g[t] + e[t] : e[t];
     return h;
+}
+
+// The split fluxes f+- = (u^2/2 +- alpha u)/2 have du-derivatives hu +- ha
+// and dalpha-derivatives +-hu, with hu = u/2 and ha = alpha/2.  A node whose
+// split-flux adjoints are Pb (of f+) and Mb (of f-) therefore receives
+// hu (Pb + Mb) + ha (Pb - Mb) in its u adjoint and adds hu (Pb - Mb) to
+// <lam, dL/dalpha>.  Applying this once per node, after summing the face
+// terms, replaces applying it to each of the ten stencil terms of a face.
+__device__ __forceinline__ double split_chain(double hu, double ha, double Pb, double Mb, double &sLa) {
+    const double d = Pb - Mb;
+    sLa += hu * d;
+    return hu * (Pb + Mb) + ha * d;
 }
 
 // Device-side scalars of the reverse sweep.
@@ -124,69 +141,25 @@ __global__ void k_ctrl_rec(Ctrl *c, StepRec *rec) {
     c->rec_step0 = c->steps;
 }
 
-// Contributions of one face to the six nodes of its stencil: the face
-// i+1/2 reads f+ at nodes i-2..i+2 and f- at i-1..i+3; node t of i-2..i+3
-// receives Fb * (dF/df+_t * (u_t+alpha)/2 + dF/df-_t * (u_t-alpha)/2).
-// Accumulates F*Fb (the face's share of <lam, L>) and Fb * sum_t u_t/2 *
-// (dF/df+_t - dF/df-_t) (its share of <lam, dL/dalpha>).  SEQ: the two sides
-// are evaluated one after the other (a scheduling barrier keeps the compiler
-// from interleaving them), which lowers the live registers.
-template <int WENO, int P, bool SEQ>
-__device__ __forceinline__ void face_vjp(const double (&fp)[6], const double (&fm)[6], const double (&uu)[6],
-                                         double Fb, double alpha, double eps, double (&c)[6], double &sL,
-                                         double &sLa) {
-    double g[5];
-    const double hp = side_grad<WENO, P, true>(fp[0], fp[1], fp[2], fp[3], fp[4], eps, g);
-    double la = 0.0;
-#pragma unroll
-    for (int t = 0; t < 5; t++) {
-        const double G = Fb * g[t];
-        c[t] = G * (0.5 * (uu[t] + alpha));
-        la += 0.5 * uu[t] * G;
-    }
-    c[5] = 0.0;
-    if (SEQ) __builtin_amdgcn_sched_barrier(0);
-    const double hm = side_grad<WENO, P, false>(fm[1], fm[2], fm[3], fm[4], fm[5], eps, g);
-#pragma unroll
-    for (int t = 0; t < 5; t++) {
-        const double G = Fb * g[t];
-        c[t + 1] += G * (0.5 * (uu[t + 1] - alpha));
-        la -= 0.5 * uu[t + 1] * G;
-    }
-    sL += (hp + hm) * Fb;
-    sLa += la;
-}
-
-// face_vjp for a face whose stencil lives in LDS: split fluxes Pp[t*st],
-// Pm[t*st] and u U[t*st] of nodes t = 0..5 (i-2..i+3), contributions written
-// to G[t*gst].  Each side loads only its own inputs, after the previous side
-// is done, so fewer values are live at once.
+// Reverse of the face i+1/2 whose split fluxes are Pp[t*st], Pm[t*st] for
+// nodes t = 0..5 (i-2..i+3), seeded with Fb: stores Fb * dF/df+ of nodes
+// 0..4 at G[t*ga] (t = 0..4) and Fb * dF/df- of nodes 1..5 at
+// G[(5+t)*ga]; returns the face flux.  The two sides are evaluated one
+// after the other (a scheduling barrier keeps the compiler from
+// interleaving them), which lowers the live registers.
 template <int WENO, int P>
-__device__ __forceinline__ void face_vjp_lds(const double *Pp, const double *Pm, const double *Up, int st, double Fb,
-                                             double alpha, double eps, double *G, int gst, double &sL, double &sLa) {
-    double g[5], c[5];
-    double la = 0.0;
-    const double hp = side_grad<WENO, P, true>(Pp[0], Pp[st], Pp[2 * st], Pp[3 * st], Pp[4 * st], eps, g);
+__device__ __forceinline__ double face_vjp_lds(const double *Pp, const double *Pm, int st, double Fb, double eps,
+                                               double *G, int ga) {
+    double g[5];
+    const double hp = side_vjp<WENO, P, true, false>(Pp[0], Pp[st], Pp[2 * st], Pp[3 * st], Pp[4 * st], eps, Fb, g);
 #pragma unroll
-    for (int t = 0; t < 5; t++) {
-        const double ut = Up[t * st];
-        const double Gt = Fb * g[t];
-        c[t] = Gt * (0.5 * (ut + alpha));
-        la += 0.5 * ut * Gt;
-    }
+    for (int t = 0; t < 5; t++) G[t * ga] = g[t];
     __builtin_amdgcn_sched_barrier(0);
-    const double hm = side_grad<WENO, P, false>(Pm[st], Pm[2 * st], Pm[3 * st], Pm[4 * st], Pm[5 * st], eps, g);
-    G[0] = c[0];
+    const double hm =
+        side_vjp<WENO, P, false, false>(Pm[st], Pm[2 * st], Pm[3 * st], Pm[4 * st], Pm[5 * st], eps, Fb, g);
 #pragma unroll
-    for (int t = 0; t < 5; t++) {
-        const double ut = Up[(t + 1) * st];
-        const double Gt = Fb * g[t];
-        const double m = Gt * (0.5 * (ut - alpha));
-        G[(t + 1) * gst] = t < 4 ? c[t + 1] + m : m;
-        la -= 0.5 * ut * Gt;
-    }
-    sL += (hp + hm) * Fb;
-    sLa += la;
+    for (int t = 0; t < 5; t++) G[(5 + t) * ga] = g[t];
+    return hp + hm;
 }
 
 // Grid of the VJP launches: extents, spacings, the number of plane ranges of
@@ -201,17 +174,22 @@ __host__ __device__ inline int range_lo(int c, int n, int nz) {
     return static_cast<int>(static_cast<long long>(c) * nz / n);
 }
 
-// LDS of one x/y VJP tile: split fluxes and u of the current plane with a halo
-// of two nodes below and three above the tile in x and y, lambda with one node
-// above, and the six node contributions of every x and y face of the tile.
+// LDS of one x/y VJP tile: split fluxes of the current plane with a halo of
+// two nodes below and three above the tile in x and y, lambda with one node
+// above, and the face terms of one axis at a time.  G holds ten arrays
+// (Fb dF/df+ of stencil nodes 0..4, then Fb dF/df- of nodes 1..5) of
+// (TY+5) rows by (TX+5) columns; the face at tile offset f of tile row or
+// column r is stored at row r+5, column f+5 for x faces and row f+5, column
+// r+5 for y faces.  Only rows and columns 5..T+4 of each array are ever
+// written: the rest stays zero, so a gather over faces f-5..f+4 of either
+// axis reads zeros for faces outside the tile and needs no bounds tests.
 template <int TX, int TY>
 struct VjpSmem {
+    static constexpr int GW = TX + 5, GA = (TY + 5) * GW, GN = 10 * GA + 5 * GW;
+    double G[GN];
     double Pp[TY + 5][TX + 5];  // f+
     double Pm[TY + 5][TX + 5];  // f-
-    double U[TY + 5][TX + 5];
     double Lm[TY + 1][TX + 1];
-    double Gx[6][TY][TX];  // Gx[t][ty][tx]: face x0+tx+1/2 to node x0+tx-2+t
-    double Gy[6][TY][TX];  // Gy[t][ty][tx]: face y0+ty+1/2 to node y0+ty-2+t
     double red[2][TX * TY / 64];
 };
 
@@ -245,7 +223,8 @@ __device__ __forceinline__ void block_partials(double sL, double sLa, double (&r
 // with Fb = (lam_{i+1} - lam_i)/d and lam = coef * ubo.  A block owns a
 // TX x TY column tile over a range of planes and computes the tile's own
 // faces x0..x0+TX-1 (+1/2) and y0..y0+TY-1 (+1/2) once each, one x and one y
-// face per thread and plane.  Contributions to nodes of a neighbouring tile
+// face per thread and plane; the x faces' terms are gathered before the y
+// faces reuse the LDS arrays.  Contributions to nodes of a neighbouring tile
 // (two below, three above in each axis) go to the exchange fields exx / exy;
 // k_vjp_z adds them.  Pass-through terms: stage 3 ubi = 2/3 ubo,
 // acc = 1/3 ubo; stage 2 ubi = 1/4 ubo, acc += 3/4 ubo; stage 1 ubi (= acc,
@@ -255,10 +234,11 @@ __global__ __launch_bounds__(TX * TY, 3) void k_vjp_xy(
     const double *__restrict__ u, const double *__restrict__ ubo, double *ubi, double *acc, const StepRec *rec,
     int stage, VjpGrid g, double eps, double *__restrict__ exx, double *__restrict__ exy,
     AdjScalars *sc, double *partials) {
-    constexpr int NT = TX * TY;
+    using Smem = VjpSmem<TX, TY>;
+    constexpr int NT = TX * TY, GW = Smem::GW, GA = Smem::GA;
     static_assert(NT % 64 == 0 && TX >= 5 && TY >= 5, "tile shape");
     static_assert(5 * (TX + TY) + TX + TY <= NT, "one halo role per thread");
-    __shared__ VjpSmem<TX, TY> sm;
+    __shared__ Smem sm;
     // the argmax slot of the stage processed before this one (reverse order
     // 3, 2, 1, 3, ...) has been used by k_vjp_tail: clear it for its next use
     if (blockIdx.x == 0 && threadIdx.x == 0) sc->idx[stage == 3 ? 0 : stage] = 0xffffffffu;
@@ -270,25 +250,33 @@ __global__ __launch_bounds__(TX * TY, 3) void k_vjp_xy(
     const int tid = threadIdx.x, tx = tid % TX, ty = tid / TX;
     const double dt = rec->dt;
     const double alpha = stage == 1 ? rec->a1 : (stage == 2 ? rec->a2 : rec->a3);
+    const double ha = 0.5 * alpha;
     const double coef = (stage == 1 ? 1.0 : (stage == 2 ? 0.25 : 2.0 / 3.0)) * dt;
     const double rdx = 1.0 / g.dx, rdy = 1.0 / g.dy;
     const unsigned plane = static_cast<unsigned>(nx) * static_cast<unsigned>(ny);
     const unsigned col = static_cast<unsigned>(x0 + tx) + static_cast<unsigned>(nx) * (y0 + ty);
+    for (int i = tid; i < Smem::GN; i += NT) sm.G[i] = 0.0;
 
-    // u halo role: 5*TY x-halo nodes (x0-2, x0-1, x0+TX..x0+TX+2), then 5*TX y-halo nodes
-    int hr = 0, hc = 0;
+    // u halo role: 5*TY x-halo nodes (x0-2, x0-1, x0+TX..x0+TX+2), then 5*TX
+    // y-halo nodes; the same thread later gathers the tile's contributions to
+    // that node (its exchange value), tile offset ot in the halo's axis
+    int hr = 0, hc = 0, ot = 0, orr = 0;
     unsigned hcol = 0;
-    const bool has_halo = tid < 5 * (TX + TY);
-    if (tid < 5 * TY) {
+    const bool xrole = tid < 5 * TY, has_halo = tid < 5 * (TX + TY);
+    if (xrole) {
         const int r = tid / 5, c = tid % 5;
         hr = r + 2;
         hc = c < 2 ? c : TX + c;
-        hcol = static_cast<unsigned>(wrapi(x0 + hc - 2, nx)) + static_cast<unsigned>(nx) * (y0 + r);
+        ot = hc - 2;
+        orr = r;
+        hcol = static_cast<unsigned>(wrapi(x0 + ot, nx)) + static_cast<unsigned>(nx) * (y0 + r);
     } else if (has_halo) {
         const int h = tid - 5 * TY, c = h % TX, q = h / TX;
         hr = q < 2 ? q : TY + q;
         hc = c + 2;
-        hcol = static_cast<unsigned>(x0 + c) + static_cast<unsigned>(nx) * wrapi(y0 + hr - 2, ny);
+        ot = hr - 2;
+        orr = c;
+        hcol = static_cast<unsigned>(x0 + c) + static_cast<unsigned>(nx) * wrapi(y0 + ot, ny);
     }
     // lambda halo role: column x0+TX (TY nodes), row y0+TY (TX nodes), last threads
     const int lt = tid - (NT - TX - TY);
@@ -306,38 +294,30 @@ __global__ __launch_bounds__(TX * TY, 3) void k_vjp_xy(
             lcol = static_cast<unsigned>(x0 + lc) + static_cast<unsigned>(nx) * wrapi(y0 + TY, ny);
         }
     }
-    // outflow role (after the faces): 5 x targets per row, then 5 y targets per column
-    int ot = 0;
-    unsigned oidx = 0;
-    if (tid < 5 * TY) {
-        const int r = tid / 5, c5 = tid % 5;
-        ot = c5 < 2 ? c5 - 2 : TX + c5 - 2;
-        oidx = static_cast<unsigned>(wrapi(x0 + ot, nx)) + static_cast<unsigned>(nx) * (y0 + r);
-    } else if (has_halo) {
-        const int h = tid - 5 * TY, cc = h % TX, q = h / TX;
-        ot = q < 2 ? q - 2 : TY + q - 2;
-        oidx = static_cast<unsigned>(x0 + cc) + static_cast<unsigned>(nx) * wrapi(y0 + ot, ny);
-    }
 
-    // sum over own faces of the contributions to target node tt (tile-relative,
-    // -2..T+2) of row / column r, in a fixed order
-    auto gather_x = [&](int r, int tt) {
-        double s = 0.0;
+    // split-flux adjoints of the node at tile offset n of row / column r from
+    // the faces n-3..n+2 of the axis whose terms G holds
+    auto gather_x = [&](int r, int n, double &Pb, double &Mb) {
+        const double *q = &sm.G[(r + 5) * GW + n + 5];
+        double p = 0.0, m = 0.0;
 #pragma unroll
-        for (int t = 0; t < 6; t++) {
-            const int f = tt + 2 - t;
-            if (f >= 0 && f < TX) s += sm.Gx[t][r][f];
+        for (int t = 0; t < 5; t++) {
+            p += q[t * GA + 2 - t];
+            m += q[(5 + t) * GA + 1 - t];
         }
-        return s;
+        Pb = p;
+        Mb = m;
     };
-    auto gather_y = [&](int r, int tt) {
-        double s = 0.0;
+    auto gather_y = [&](int r, int n, double &Pb, double &Mb) {
+        const double *q = &sm.G[(n + 5) * GW + r + 5];
+        double p = 0.0, m = 0.0;
 #pragma unroll
-        for (int t = 0; t < 6; t++) {
-            const int f = tt + 2 - t;
-            if (f >= 0 && f < TY) s += sm.Gy[t][f][r];
+        for (int t = 0; t < 5; t++) {
+            p += q[t * GA + (2 - t) * GW];
+            m += q[(5 + t) * GA + (1 - t) * GW];
         }
-        return s;
+        Pb = p;
+        Mb = m;
     };
 
     double sL = 0.0, sLa = 0.0;
@@ -346,6 +326,7 @@ __global__ __launch_bounds__(TX * TY, 3) void k_vjp_xy(
     double ub_next = ubo[col + plane * static_cast<unsigned>(z0)];
     double lh_next = has_lh ? ubo[lcol + plane * static_cast<unsigned>(z0)] : 0.0;
     double ac_next = stage == 3 ? 0.0 : acc[col + plane * static_cast<unsigned>(z0)];
+    double *const Gf = &sm.G[(ty + 5) * GW + tx + 5];  // this thread's face slot
     for (int k = z0; k < z1; k++) {
         const double uv = u_next, hv = h_next, ub = ub_next, lh = lh_next, ac = ac_next;
         if (k + 1 < z1) {
@@ -361,7 +342,6 @@ __global__ __launch_bounds__(TX * TY, 3) void k_vjp_xy(
             lf_split(uv, alpha, fp, fm);
             sm.Pp[ty + 2][tx + 2] = fp;
             sm.Pm[ty + 2][tx + 2] = fm;
-            sm.U[ty + 2][tx + 2] = uv;
             sm.Lm[ty][tx] = coef * ub;
         }
         if (has_halo) {
@@ -369,25 +349,37 @@ __global__ __launch_bounds__(TX * TY, 3) void k_vjp_xy(
             lf_split(hv, alpha, hp, hm);
             sm.Pp[hr][hc] = hp;
             sm.Pm[hr][hc] = hm;
-            sm.U[hr][hc] = hv;
         }
         if (has_lh) sm.Lm[lr][lc] = coef * lh;
         __syncthreads();
 
+        const unsigned pk = plane * static_cast<unsigned>(k);
         {  // x face x0+tx+1/2: nodes tile-x tx-2..tx+3 = LDS cols tx..tx+5
             const double Fb = (sm.Lm[ty][tx + 1] - sm.Lm[ty][tx]) * rdx;
-            face_vjp_lds<WENO, P>(&sm.Pp[ty + 2][tx], &sm.Pm[ty + 2][tx], &sm.U[ty + 2][tx], 1, Fb, alpha, eps,
-                                  &sm.Gx[0][ty][tx], TX * TY, sL, sLa);
-        }
-        __builtin_amdgcn_sched_barrier(0);
-        {  // y face y0+ty+1/2: LDS rows ty..ty+5
-            const double Fb = (sm.Lm[ty + 1][tx] - sm.Lm[ty][tx]) * rdy;
-            face_vjp_lds<WENO, P>(&sm.Pp[ty][tx + 2], &sm.Pm[ty][tx + 2], &sm.U[ty][tx + 2], TX + 5, Fb, alpha,
-                                  eps, &sm.Gy[0][ty][tx], TX * TY, sL, sLa);
+            sL += face_vjp_lds<WENO, P>(&sm.Pp[ty + 2][tx], &sm.Pm[ty + 2][tx], 1, Fb, eps, Gf, GA) * Fb;
         }
         __syncthreads();
+        double Px, Mx;
+        gather_x(ty, tx, Px, Mx);
+        if (xrole) {
+            double Po, Mo;
+            gather_x(orr, ot, Po, Mo);
+            exx[hcol + pk] = split_chain(0.5 * hv, ha, Po, Mo, sLa);
+        }
+        __syncthreads();
+        {  // y face y0+ty+1/2: LDS rows ty..ty+5
+            const double Fb = (sm.Lm[ty + 1][tx] - sm.Lm[ty][tx]) * rdy;
+            sL += face_vjp_lds<WENO, P>(&sm.Pp[ty][tx + 2], &sm.Pm[ty][tx + 2], TX + 5, Fb, eps, Gf, GA) * Fb;
+        }
+        __syncthreads();
+        double Py, My;
+        gather_y(tx, ty, Py, My);
+        if (has_halo && !xrole) {
+            double Po, Mo;
+            gather_y(orr, ot, Po, Mo);
+            exy[hcol + pk] = split_chain(0.5 * hv, ha, Po, Mo, sLa);
+        }
 
-        const unsigned pk = plane * static_cast<unsigned>(k);
         const unsigned o = col + pk;
         double pass;
         if (stage == 3) {
@@ -399,26 +391,23 @@ __global__ __launch_bounds__(TX * TY, 3) void k_vjp_xy(
         } else {
             pass = ac + ub;
         }
-        ubi[o] = pass + (gather_x(ty, tx) + gather_y(tx, ty));
+        ubi[o] = pass + split_chain(0.5 * uv, ha, Px + Py, Mx + My, sLa);
         if (fabs(uv) == alpha) atomicMin(&sc->idx[stage - 1], o);
-        if (tid < 5 * TY)
-            exx[oidx + pk] = gather_x(tid / 5, ot);
-        else if (has_halo)
-            exy[oidx + pk] = gather_y((tid - 5 * TY) % TX, ot);
     }
     block_partials<NT>(sL, sLa, sm.red, partials);
 }
 
 // z part of the stage VJP, one thread per column and z chunk (lanes along x):
 // marches the chunk's own z faces z0..z1-1 (+1/2) with register queues of the
-// split fluxes and u of planes k-2..k+3 and adds, for each node of the chunk,
-// the contributions of its own z faces and the x/y exchange values k_vjp_xy
-// left for it (x offsets 0..2 and TX-2..TX-1 within its tile, the same in
-// y).  Node k-2 is finished once its last own face, k+1/2, is done;
-// contributions to nodes of the chunks below (z0-2, z0-1) and above
-// (z1..z1+2) go to exz, which k_vjp_tail adds.
+// split fluxes and u/2 of planes k-2..k+3 and split-flux adjoint
+// accumulators, and adds, for each node of the chunk, the contribution of
+// its own z faces and the x/y exchange values k_vjp_xy left for it (x
+// offsets 0..2 and TX-2..TX-1 within its tile, the same in y).  Node k-2 is
+// finished once its last own face, k+1/2, is done; contributions to nodes
+// of the chunks below (z0-2, z0-1) and above (z1..z1+2) go to exz, which
+// k_vjp_tail adds.
 template <int WENO, int P, int TX, int TY>
-__global__ __launch_bounds__(256, 4) void k_vjp_z(const double *__restrict__ u, const double *__restrict__ ubo,
+__global__ __launch_bounds__(256, 3) void k_vjp_z(const double *__restrict__ u, const double *__restrict__ ubo,
                                                double *ubi, const StepRec *rec, int stage, VjpGrid g, double eps,
                                                const double *__restrict__ exx, const double *__restrict__ exy,
                                                double *__restrict__ exz, double *partials) {
@@ -434,56 +423,74 @@ __global__ __launch_bounds__(256, 4) void k_vjp_z(const double *__restrict__ u, 
         const bool ix = x % TX <= 2 || x % TX >= TX - 2, iy = y % TY <= 2 || y % TY >= TY - 2;
         const double dt = rec->dt;
         const double alpha = stage == 1 ? rec->a1 : (stage == 2 ? rec->a2 : rec->a3);
+        const double ha = 0.5 * alpha;
         const double coef = (stage == 1 ? 1.0 : (stage == 2 ? 0.25 : 2.0 / 3.0)) * dt;
         const double rdz = 1.0 / g.dz;
         auto zo = [&](int k) { return col + plane * static_cast<unsigned>(wrapi(k, nz)); };
-        auto finish = [&](int k, double zc) {
+        // ubi, exx and exy of the next node to finish, loaded one face ahead
+        double fb = 0.0, fx = 0.0, fy = 0.0;
+        auto fetch = [&](int k) {
             const unsigned o = col + plane * static_cast<unsigned>(k);
-            double s = zc;
-            if (ix) s += exx[o];
-            if (iy) s += exy[o];
-            ubi[o] += s;
+            fb = ubi[o];
+            fx = ix ? exx[o] : 0.0;
+            fy = iy ? exy[o] : 0.0;
         };
-        double fpq[6], fmq[6], uq[6], az[6];
+        auto finish = [&](int k, double zc) {
+            ubi[col + plane * static_cast<unsigned>(k)] = fb + ((zc + fx) + fy);
+        };
+        // queues over nodes k-2..k+3; ap / am: adjoints of f+ / f- of nodes k-2..k+2
+        double fpq[6], fmq[6], hq[6], ap[5], am[5];
 #pragma unroll
         for (int q = 0; q < 5; q++) {
-            uq[q] = u[zo(z0 - 2 + q)];
-            lf_split(uq[q], alpha, fpq[q], fmq[q]);
+            const double v = u[zo(z0 - 2 + q)];
+            lf_split(v, alpha, fpq[q], fmq[q]);
+            hq[q] = 0.5 * v;
+            ap[q] = 0.0;
+            am[q] = 0.0;
         }
-#pragma unroll
-        for (int q = 0; q < 6; q++) az[q] = 0.0;
         double l0 = coef * ubo[zo(z0)];
         double u_next = u[zo(z0 + 3)], ub_next = ubo[zo(z0 + 1)];
         for (int k = z0; k < z1; k++) {
-            uq[5] = u_next;
+            const double v = u_next;
             const double l1 = coef * ub_next;
             if (k + 1 < z1) {
                 u_next = u[zo(k + 4)];
                 ub_next = ubo[zo(k + 2)];
             }
-            lf_split(uq[5], alpha, fpq[5], fmq[5]);
-            double c[6];
-            face_vjp<WENO, P, false>(fpq, fmq, uq, (l1 - l0) * rdz, alpha, eps, c, sL, sLa);
-#pragma unroll
-            for (int t = 0; t < 6; t++) az[t] += c[t];
+            lf_split(v, alpha, fpq[5], fmq[5]);
+            hq[5] = 0.5 * v;
+            const double Fb = (l1 - l0) * rdz;
+            double gm[5];
+            const double hp = side_vjp<WENO, P, true, true>(fpq[0], fpq[1], fpq[2], fpq[3], fpq[4], eps, Fb, ap);
+            const double hm =
+                side_vjp<WENO, P, false, false>(fmq[1], fmq[2], fmq[3], fmq[4], fmq[5], eps, Fb, gm);
+            sL += (hp + hm) * Fb;
+            const double c = split_chain(hq[0], ha, ap[0], am[0], sLa);
             if (k - 2 >= z0)
-                finish(k - 2, az[0]);
+                finish(k - 2, c);
             else
-                exz[zo(k - 2)] = az[0];
+                exz[zo(k - 2)] = c;
+            if (k - 1 >= z0) fetch(k - 1);
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                ap[q] = ap[q + 1];
+                am[q] = am[q + 1] + gm[q];
+            }
+            ap[4] = 0.0;
+            am[4] = gm[4];
 #pragma unroll
             for (int q = 0; q < 5; q++) {
-                az[q] = az[q + 1];
                 fpq[q] = fpq[q + 1];
                 fmq[q] = fmq[q + 1];
-                uq[q] = uq[q + 1];
+                hq[q] = hq[q + 1];
             }
-            az[5] = 0.0;
             l0 = l1;
         }
-        finish(z1 - 2, az[0]);
-        finish(z1 - 1, az[1]);
+        finish(z1 - 2, split_chain(hq[0], ha, ap[0], am[0], sLa));
+        fetch(z1 - 1);
+        finish(z1 - 1, split_chain(hq[1], ha, ap[1], am[1], sLa));
 #pragma unroll
-        for (int q = 2; q < 5; q++) exz[zo(z1 + q - 2)] = az[q];
+        for (int q = 2; q < 5; q++) exz[zo(z1 + q - 2)] = split_chain(hq[q], ha, ap[q], am[q], sLa);
     }
     block_partials<256>(sL, sLa, red, partials);
 }

@@ -434,9 +434,24 @@ def weno7_cases(steps: int = 10, warmup: int = 2) -> dict:
     return out
 
 
+def adjoint_trace(n: int = 128, steps: int = 100) -> dict:
+    """One store-all adjoint of a `steps`-step n^3 WENO5-Z run: the workload
+    of the adjoint kernel trace (forward stages, then per stage k_vjp_xy,
+    k_vjp_z and k_vjp_tail)."""
+    u0 = solvers.ic_fourier(n, seed=5)
+    s = solvers.PeriodicBurgers(n, scheme="Z2")
+    s.set_initial(u0, tend=float("inf"))
+    s.steps(steps)
+    T = s.state().t
+    adj = solvers.BurgersAdjoint(n, scheme="Z2", max_steps=steps + 8, every=steps + 8, stream=s.stream)
+    J, _, nst = adj(u0, 0.9 * u0, T)
+    return {"study": "adjoint_trace", "grid": n, "steps": nst, "J": J}
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
-    ap.add_argument("study", choices=["batched", "fp32", "adjoint", "reference", "heat", "slab", "weno7"])
+    ap.add_argument("study",
+                    choices=["batched", "fp32", "adjoint", "adjoint_trace", "reference", "heat", "slab", "weno7"])
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--n", type=int, default=512)
     args = ap.parse_args(argv)
@@ -452,6 +467,8 @@ def main(argv=None) -> int:
         r = weno7_cases()
     elif args.study == "adjoint":
         r = adjoint(args.n if args.n != 512 else 128)
+    elif args.study == "adjoint_trace":
+        r = adjoint_trace(args.n if args.n != 512 else 128, args.steps if args.steps != 200 else 100)
     else:
         r = fp32(args.n)
     print(json.dumps(r), flush=True)

@@ -61,6 +61,21 @@ def test_adjoint_vs_oracle_and_fd(hipdev, scheme, tend):
         assert abs(fd6 - ad) <= 1e-6 * gnorm, (fd6, ad, gnorm)
 
 
+# (48, 32, 8): one z chunk, whose lower and upper boundaries are the same
+# periodic plane; (48, 32, 40): chunks of 5 planes, the shortest allowed.
+# x has the finest spacing, which sets the CFL step.
+@pytest.mark.parametrize("shape", [(48, 32, 8), (48, 32, 40)])
+def test_adjoint_ragged_grids_vs_oracle(hipdev, shape):
+    u0 = solvers.ic_fourier(shape, seed=11)
+    target = solvers.ic_fourier(shape, seed=11) + 1e-2 * solvers.ic_fourier(shape, seed=12)
+    adj = solvers.BurgersAdjoint(shape, scheme="Z2", max_steps=100)
+    J, g, steps = adj(u0, target, 0.25)
+    Jo, go, so = O.burgers_adjoint(u0, target, (adj.dx, adj.dy, adj.dz), 0.4, 0.0, 0.25, **ORC["Z2"])
+    assert steps == so
+    assert J == pytest.approx(Jo, rel=1e-12)
+    assert np.max(np.abs(g - go)) <= 1e-9 * np.max(np.abs(go))
+
+
 @pytest.mark.parametrize("every", [1, 3, 7, 1000])
 def test_checkpoint_interval_does_not_change_gradient(hipdev, every):
     n = 32
