@@ -552,6 +552,17 @@ int mgad_init3d_heat(int ic, double *u0, double dx, double dy, double dz,
 int mgad_heat_error_norms(const double *u, double t, double dx, double dy,
                           double dz, unsigned int nx, unsigned int ny,
                           unsigned int nz, double *out);
+/* Init_subdomain, MultiGPU/Burgers3d_Baseline/Tools.c:181-199 (also
+ * Diffusion3d_Baseline/Tools.c:180-198): copies planes n*_Nz .. n*_Nz+_Nz+5
+ * of the global Nx x Ny x (P*_Nz+6) field h_q into the slab h_s_q
+ * (Nx x Ny x (_Nz+6): 3 ghost planes each side).  Host arrays. */
+int mgad_init_subdomain(const double *h_q, double *h_s_q, unsigned int n,
+                        unsigned int Nx, unsigned int Ny, unsigned int _Nz);
+/* Merge_domains, MultiGPU/Burgers3d_Baseline/Tools.c:204-223 (also
+ * Diffusion3d_Baseline/Tools.c:203-222): copies the slab's own planes
+ * 3 .. _Nz+2 into planes n*_Nz+3 .. n*_Nz+_Nz+2 of h_q.  Host arrays. */
+int mgad_merge_domains(const double *h_s_q, double *h_q, unsigned int n,
+                       unsigned int Nx, unsigned int Ny, unsigned int _Nz);
 
 #ifdef __cplusplus
 }

@@ -31,6 +31,7 @@ EXPORTS = (
     "mgad_adjoint_workspace_bytes", "mgad_burgers_adjoint",
     "mgad_burgers_ref_steps", "mgad_heat7_fe_steps", "mgad_heat13_rk3_steps", "mgad_check_division", "mgad_ic_fourier", "mgad_ic_gauss_noise",
     "mgad_save_binary3d", "mgad_load_binary3d", "mgad_init3d_burgers", "mgad_init3d_heat", "mgad_heat_error_norms",
+    "mgad_init_subdomain", "mgad_merge_domains",
     "mgad_slab_exchange", "mgad_burgers_slab_init", "mgad_burgers_slab_steps",
     "Compute_Adv_x_Async", "Compute_Adv_y_Async", "Compute_Adv_z_Async", "Compute_sspRK3Async",
     "CopyBoundaryRegionToGhostCellAsync", "CopyGhostCellToBoundaryRegionAsync",
@@ -191,6 +192,10 @@ def lib():
             f.restype = I
         L.mgad_heat_error_norms.argtypes = [VP, D, D, D, D, U, U, U, VP]
         L.mgad_heat_error_norms.restype = I
+        for name in ("mgad_init_subdomain", "mgad_merge_domains"):
+            f = getattr(L, name)
+            f.argtypes = [VP, VP, U, U, U, U]
+            f.restype = I
         L.mgad_ic_fourier.argtypes = [VP, U, U, U, D, D, ctypes.c_uint64, I, I, D, D]
         L.mgad_ic_fourier.restype = I
         L.mgad_ic_gauss_noise.argtypes = [VP, U, U, U, D, D, ctypes.c_uint64]

@@ -3,6 +3,7 @@
 // reference's drivers do it.
 #include <cmath>
 #include <cstdio>
+#include <cstring>
 #include <vector>
 
 #include "mgad_common.hpp"
@@ -174,5 +175,34 @@ extern "C" int mgad_heat_error_norms(const double *u, double t, double dx, doubl
     out[0] = dx * dy * dz * l1;
     out[1] = std::sqrt(dx * dy * dz * l2);
     out[2] = linf;
+    return MGAD_OK;
+}
+
+// Init_subdomain and Merge_domains, MultiGPU/Burgers3d_Baseline/Tools.c:181-223
+// (the same in Diffusion3d_Baseline/Tools.c:180-222): subdomain n of a field
+// split along z into slabs of _Nz own planes with 3 ghost planes each side.
+// The global field is Nx x Ny x (P _Nz + 6); subdomain n holds its planes
+// n _Nz .. n _Nz + _Nz + 5.  Init copies all _Nz + 6 planes out, Merge copies
+// the _Nz own planes (3 .. _Nz + 2) back.  Each plane is a contiguous run of
+// Nx Ny values, so both are one memcpy per slab.
+extern "C" int mgad_init_subdomain(const double *h_q, double *h_s_q, unsigned int n, unsigned int Nx,
+                                   unsigned int Ny, unsigned int _Nz) {
+    if (!h_q || !h_s_q || _Nz == 0 || _Nz > (1u << 30) || !dims_ok(Nx, Ny, _Nz + 6)) {
+        set_error(MGAD_ERR_INVALID_ARG);
+        return MGAD_ERR_INVALID_ARG;
+    }
+    const size_t xy = static_cast<size_t>(Nx) * Ny;
+    std::memcpy(h_s_q, h_q + xy * (static_cast<size_t>(n) * _Nz), sizeof(double) * xy * (_Nz + 6));
+    return MGAD_OK;
+}
+
+extern "C" int mgad_merge_domains(const double *h_s_q, double *h_q, unsigned int n, unsigned int Nx,
+                                  unsigned int Ny, unsigned int _Nz) {
+    if (!h_q || !h_s_q || _Nz == 0 || _Nz > (1u << 30) || !dims_ok(Nx, Ny, _Nz + 6)) {
+        set_error(MGAD_ERR_INVALID_ARG);
+        return MGAD_ERR_INVALID_ARG;
+    }
+    const size_t xy = static_cast<size_t>(Nx) * Ny;
+    std::memcpy(h_q + xy * (static_cast<size_t>(n) * _Nz + 3), h_s_q + xy * 3, sizeof(double) * xy * _Nz);
     return MGAD_OK;
 }

@@ -59,3 +59,34 @@ def heat_error_norms(u: np.ndarray, t: float, dx: float, dy: float, dz: float):
     _capi.check(_capi.lib().mgad_heat_error_norms(u.ctypes.data_as(ctypes.c_void_p), t, dx, dy, dz, *_shape(u),
                                                   out.ctypes.data_as(ctypes.c_void_p)), "mgad_heat_error_norms")
     return tuple(out)
+
+
+def init_subdomain(h_q: np.ndarray, n: int, nz_own: int) -> np.ndarray:
+    """Init_subdomain (MultiGPU/Burgers3d_Baseline/Tools.c:181-199): slab n,
+    planes n*nz_own .. n*nz_own + nz_own + 5 of the global field h_q
+    (shape (P*nz_own + 6, Ny, Nx))."""
+    h_q = np.ascontiguousarray(h_q, dtype=np.float64)
+    _, ny, nx = h_q.shape
+    if (n + 1) * nz_own + 6 > h_q.shape[0]:
+        raise ValueError("slab n reaches past the global field")
+    out = np.empty((nz_own + 6, ny, nx))
+    _capi.check(_capi.lib().mgad_init_subdomain(h_q.ctypes.data_as(ctypes.c_void_p),
+                                                out.ctypes.data_as(ctypes.c_void_p), n, nx, ny, nz_own),
+                "mgad_init_subdomain")
+    return out
+
+
+def merge_domains(h_s_q: np.ndarray, h_q: np.ndarray, n: int) -> None:
+    """Merge_domains (MultiGPU/Burgers3d_Baseline/Tools.c:204-223): writes
+    the own planes 3 .. nz_own+2 of slab n into planes n*nz_own+3 .. of h_q,
+    in place (h_q must be C-contiguous float64)."""
+    h_s_q = np.ascontiguousarray(h_s_q, dtype=np.float64)
+    nz_own = h_s_q.shape[0] - 6
+    if h_q.dtype != np.float64 or not h_q.flags.c_contiguous or h_q.shape[1:] != h_s_q.shape[1:]:
+        raise ValueError("h_q must be a C-contiguous float64 field of the slab's Nx, Ny")
+    if nz_own < 1 or (n + 1) * nz_own + 6 > h_q.shape[0]:
+        raise ValueError("slab n reaches past the global field")
+    _, ny, nx = h_q.shape
+    _capi.check(_capi.lib().mgad_merge_domains(h_s_q.ctypes.data_as(ctypes.c_void_p),
+                                               h_q.ctypes.data_as(ctypes.c_void_p), n, nx, ny, nz_own),
+                "mgad_merge_domains")

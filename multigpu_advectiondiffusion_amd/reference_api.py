@@ -12,6 +12,8 @@ from __future__ import annotations
 
 import ctypes
 
+import numpy as np
+
 from . import _capi
 from .hip import DeviceArray, Stream
 
@@ -194,3 +196,27 @@ def CopyGhostCellToBoundaryRegionAsync(numBlocks, threadsPerBlock, aStream, d_s_
                                        pitch, gc_pitch, side) -> None:
     _call("CopyGhostCellToBoundaryRegionAsync", _dim(numBlocks), _dim(threadsPerBlock), _stream(aStream),
           _ptr(d_s_q), _ptr(d_recv_buffer), Nx, Ny, _Nz, pitch, gc_pitch, side)
+
+
+def _host(a: np.ndarray, n: int) -> int:
+    if not isinstance(a, np.ndarray) or a.dtype != np.float64 or not a.flags.c_contiguous or a.size < n:
+        raise ValueError(f"need a C-contiguous float64 host array of at least {n} values")
+    return a.ctypes.data
+
+
+def Init_subdomain(h_q, h_s_q, n, Nx, Ny, _Nz) -> None:
+    """MultiGPU/Burgers3d_Baseline/Tools.c:181-199, host arrays: h_s_q gets
+    planes n*_Nz .. n*_Nz+_Nz+5 of h_q."""
+    xy = Nx * Ny
+    _host(h_q, xy * ((n + 1) * _Nz + 6))
+    _host(h_s_q, xy * (_Nz + 6))
+    _call("mgad_init_subdomain", h_q.ctypes.data, h_s_q.ctypes.data, n, Nx, Ny, _Nz)
+
+
+def Merge_domains(h_s_q, h_q, n, Nx, Ny, _Nz) -> None:
+    """MultiGPU/Burgers3d_Baseline/Tools.c:204-223, host arrays: planes
+    3.._Nz+2 of h_s_q go to planes n*_Nz+3.. of h_q."""
+    xy = Nx * Ny
+    _host(h_s_q, xy * (_Nz + 6))
+    _host(h_q, xy * ((n + 1) * _Nz + 6))
+    _call("mgad_merge_domains", h_s_q.ctypes.data, h_q.ctypes.data, n, Nx, Ny, _Nz)
