@@ -42,12 +42,34 @@ struct SweepRange {
     int b0, nb, l0, l1, zero_first;
 };
 
-// AXIS 0: x lines (rows), 1: y lines, 2: z lines.  MODE 0 writes Lu, 1 subtracts.
+// Face flux with the 13/12 terms supplied; fd (the same for the whole wave)
+// selects the shortened weight divisions, whose results equal IEEE division
+// when every node read is at most 2^40 in magnitude and eps is in range
+// (fast_division_ok).
+template <int WENO, int P>
+__device__ __forceinline__ double face_fd(bool fd, double v0, double v1, double v2, double v3, double v4, double w0,
+                                          double w1, double w2, double w3, double w4, double Sp0, double Sp1,
+                                          double Sp2, double Sm0, double Sm1, double Sm2, double eps) {
+    if (fd)
+        return weno_face_s<WENO, P, true>(v0, v1, v2, v3, v4, w0, w1, w2, w3, w4, Sp0, Sp1, Sp2, Sm0, Sm1, Sm2, eps);
+    return weno_face_s<WENO, P, false>(v0, v1, v2, v3, v4, w0, w1, w2, w3, w4, Sp0, Sp1, Sp2, Sm0, Sm1, Sm2, eps);
+}
+
+// True for the whole wave when no active lane has seen a node above 2^40 in
+// magnitude (or NaN): a scalar branch, no divergence.
+__device__ __forceinline__ bool wave_all(bool ok) { return __builtin_amdgcn_ballot_w64(!ok) == 0; }
+
+// AXIS 0: x lines (rows), 1: y lines, 2: z lines.  AXIS 0 writes Lu, 1 and 2 subtract.
 // px is the row pitch in elements (px >= nx; px = nx for unpitched arrays).
+// Along its line chunk a thread keeps the split fluxes of nodes i-3..i+3 and
+// the 13/12 terms of the sub-stencil centres in register queues, so each
+// node's terms are computed once, not once per face that reads them.
+// Lu -/= (F_{i+1/2} - F_{i-1/2})/d is div_by with rd = RN(1/d).  The next
+// node's u and Lu are loaded one face ahead.
 template <int AXIS, int WENO, int P, bool LOCAL>
 __global__ __launch_bounds__(256) void k_adv_sweep(const double *__restrict__ u,
                                                    double *__restrict__ Lu, int nx, int ny, int nz,
-                                                   int px, double d, double eps, double alpha,
+                                                   int px, double d, double rd, double eps, double alpha,
                                                    SweepRange r, int nchunks) {
     const int na = (AXIS == 0 ? ny : nx) - 6;  // first transverse coordinate count (3..n-4)
     const long gid = static_cast<long>(blockIdx.x) * 256 + threadIdx.x;
@@ -75,38 +97,57 @@ __global__ __launch_bounds__(256) void k_adv_sweep(const double *__restrict__ u,
     const int i1 = min(i0 + kChunk, r.l1);
     if (i0 >= i1) return;
     const bool low_face = i0 > r.l0 || !r.zero_first;
+    const bool eps_ok = fast_division_ok<WENO>(0.0, eps);
 
-    auto split = [&](double v, double &fp, double &fm) {
+    bool ok = true;  // every node read so far is at most 2^40 in magnitude
+    auto take = [&](double v, double &fp, double &fm) {
+        ok = ok && fabs(v) <= 0x1p40;
         if (LOCAL)
             lf_split_local(v, fp, fm);
         else
             lf_split(v, alpha, fp, fm);
     };
+    auto load = [&](int i, double &fp, double &fm) { take(u[base + st * static_cast<unsigned>(i)], fp, fm); };
     // queue: fp[q], fm[q] hold node i-3+q for the face being computed (node
-    // i0-3 is read only when the chunk's low face is computed)
+    // i0-3 is read only when the chunk's low face is computed); sp_c / sm_c
+    // are the 13/12 terms of f+ / f- centred on queue node c
     double fp[7], fm[7];
     fp[0] = fm[0] = 0.0;
-    if (low_face) split(u[base + st * static_cast<unsigned>(i0 - 3)], fp[0], fm[0]);
+    if (low_face) load(i0 - 3, fp[0], fm[0]);
 #pragma unroll
-    for (int q = 1; q < 6; q++) split(u[base + st * static_cast<unsigned>(i0 - 3 + q)], fp[q], fm[q]);
+    for (int q = 1; q < 6; q++) load(i0 - 3 + q, fp[q], fm[q]);
+    double sp2 = sterm(fp[1], fp[2], fp[3]), sp3 = sterm(fp[2], fp[3], fp[4]);
+    double sm3 = sterm(fm[2], fm[3], fm[4]), sm4 = sterm(fm[3], fm[4], fm[5]);
     double fu_old = 0.0;
     if (low_face)
-        fu_old = weno_face<WENO, P>(fp[0], fp[1], fp[2], fp[3], fp[4], fm[1], fm[2], fm[3], fm[4],
-                                    fm[5], eps);
+        fu_old = face_fd<WENO, P>(eps_ok && wave_all(ok), fp[0], fp[1], fp[2], fp[3], fp[4], fm[1], fm[2], fm[3],
+                                  fm[4], fm[5], sterm(fp[0], fp[1], fp[2]), sp2, sp3, sterm(fm[1], fm[2], fm[3]),
+                                  sm3, sm4, eps);
+    // u of node i+3 and Lu of node i are loaded one node ahead
+    double u_next = u[base + st * static_cast<unsigned>(i0 + 3)];
+    double lu_next = AXIS == 0 ? 0.0 : Lu[base + st * static_cast<unsigned>(i0)];
     for (int i = i0; i < i1; i++) {
-        split(u[base + st * static_cast<unsigned>(i + 3)], fp[6], fm[6]);
-        const double fu = weno_face<WENO, P>(fp[1], fp[2], fp[3], fp[4], fp[5], fm[2], fm[3],
-                                             fm[4], fm[5], fm[6], eps);
+        const double v = u_next, lu = lu_next;
+        if (i + 1 < i1) {
+            u_next = u[base + st * static_cast<unsigned>(i + 4)];
+            if (AXIS != 0) lu_next = Lu[base + st * static_cast<unsigned>(i + 1)];
+        }
+        take(v, fp[6], fm[6]);
+        const double sp4 = sterm(fp[3], fp[4], fp[5]), sm5 = sterm(fm[4], fm[5], fm[6]);
+        const double fu = face_fd<WENO, P>(eps_ok && wave_all(ok), fp[1], fp[2], fp[3], fp[4], fp[5], fm[2], fm[3],
+                                           fm[4], fm[5], fm[6], sp2, sp3, sp4, sm3, sm4, sm5, eps);
         const unsigned o = base + st * static_cast<unsigned>(i);
-        if (AXIS == 0)
-            Lu[o] = -(fu - fu_old) / d;
-        else
-            Lu[o] -= (fu - fu_old) / d;
+        const double q = div_by(fu - fu_old, d, rd);
+        Lu[o] = AXIS == 0 ? -q : lu - q;
         fu_old = fu;
+        sp2 = sp3;
+        sp3 = sp4;
+        sm3 = sm4;
+        sm4 = sm5;
 #pragma unroll
-        for (int q = 0; q < 6; q++) {
-            fp[q] = fp[q + 1];
-            fm[q] = fm[q + 1];
+        for (int q2 = 0; q2 < 6; q2++) {
+            fp[q2] = fp[q2 + 1];
+            fm[q2] = fm[q2 + 1];
         }
     }
 }
@@ -123,8 +164,8 @@ constexpr int kXRowNodes = 63;
 
 template <int WENO, int P, bool LOCAL>
 __global__ __launch_bounds__(256) void k_adv_x_rows(const double *__restrict__ u, double *__restrict__ Lu, int nx,
-                                                    int ny, int px, double d, double eps, double alpha, int b0,
-                                                    int nb) {
+                                                    int ny, int px, double d, double rd, double eps, double alpha,
+                                                    int b0, int nb) {
     __shared__ double2 S[4][69];  // (f+, f-) of nodes i0-3 .. i0+65
     __shared__ double F[4][64];   // faces i0-1/2 .. i0+62+1/2
     const int lane = threadIdx.x & 63, r = threadIdx.x >> 6;
@@ -135,8 +176,10 @@ __global__ __launch_bounds__(256) void k_adv_x_rows(const double *__restrict__ u
     const long lc = live ? line : nlines - 1;  // rows past the end load a valid row and store nothing
     const int j = 3 + static_cast<int>(lc % (ny - 6)), b = b0 + static_cast<int>(lc / (ny - 6));
     const unsigned base = static_cast<unsigned>(px) * j + static_cast<unsigned>(px) * ny * b;
+    bool ok = true;  // the nodes this lane loads are at most 2^40 in magnitude
     auto split = [&](double v) {
         double fp, fm;
+        ok = ok && fabs(v) <= 0x1p40;
         if (LOCAL)
             lf_split_local(v, fp, fm);
         else
@@ -147,16 +190,20 @@ __global__ __launch_bounds__(256) void k_adv_x_rows(const double *__restrict__ u
     // node 2, which is 0, so any valid node will do; nodes past the end clamp
     S[r][lane] = split(u[base + min(max(i0 - 3 + lane, 0), nx - 1)]);
     if (lane < 5) S[r][64 + lane] = split(u[base + min(i0 + 61 + lane, nx - 1)]);
+    // one wave per row: the row's fast-division test is wave-uniform
+    const bool fd = fast_division_ok<WENO>(0.0, eps) && __builtin_amdgcn_ballot_w64(!ok) == 0;
     __syncthreads();
     {
         const double2 a0 = S[r][lane], a1 = S[r][lane + 1], a2 = S[r][lane + 2], a3 = S[r][lane + 3],
                       a4 = S[r][lane + 4], a5 = S[r][lane + 5];
-        const double f = weno_face<WENO, P>(a0.x, a1.x, a2.x, a3.x, a4.x, a1.y, a2.y, a3.y, a4.y, a5.y, eps);
+        const double f =
+            fd ? weno_face2<WENO, P, true>(a0.x, a1.x, a2.x, a3.x, a4.x, a1.y, a2.y, a3.y, a4.y, a5.y, eps)
+               : weno_face2<WENO, P, false>(a0.x, a1.x, a2.x, a3.x, a4.x, a1.y, a2.y, a3.y, a4.y, a5.y, eps);
         F[r][lane] = (lane == 0 && i0 == 2) ? 0.0 : f;
     }
     __syncthreads();
     const int i = i0 + lane;
-    if (live && lane < kXRowNodes && i < nx - 3) Lu[base + i] = -(F[r][lane + 1] - F[r][lane]) / d;
+    if (live && lane < kXRowNodes && i < nx - 3) Lu[base + i] = -div_by(F[r][lane + 1] - F[r][lane], d, rd);
 }
 
 // Compute_Laplace_Async arithmetic, kernels.cu:474-476: Lu += 13-point operator.
@@ -234,10 +281,10 @@ void launch_sweep_t(hipStream_t s, unsigned nx, unsigned ny, unsigned nz, unsign
                         static_cast<unsigned>((static_cast<long>(ny - 6) * r.nb + 3) / 4));
         if (sc.lf == MGAD_LF_LOCAL)
             hipLaunchKernelGGL((k_adv_x_rows<WENO, P, true>), grid, dim3(256), 0, s, u, Lu, (int)nx, (int)ny, (int)px,
-                               d, sc.eps, sc.alpha, r.b0, r.nb);
+                               d, 1.0 / d, sc.eps, sc.alpha, r.b0, r.nb);
         else
             hipLaunchKernelGGL((k_adv_x_rows<WENO, P, false>), grid, dim3(256), 0, s, u, Lu, (int)nx, (int)ny,
-                               (int)px, d, sc.eps, sc.alpha, r.b0, r.nb);
+                               (int)px, d, 1.0 / d, sc.eps, sc.alpha, r.b0, r.nb);
         return;
     }
     const long na = (AXIS == 0 ? ny : nx) - 6;
@@ -246,10 +293,10 @@ void launch_sweep_t(hipStream_t s, unsigned nx, unsigned ny, unsigned nz, unsign
     const unsigned blocks = static_cast<unsigned>((total + 255) / 256);
     if (sc.lf == MGAD_LF_LOCAL)
         hipLaunchKernelGGL((k_adv_sweep<AXIS, WENO, P, true>), dim3(blocks), dim3(256), 0, s, u, Lu,
-                           (int)nx, (int)ny, (int)nz, (int)px, d, sc.eps, sc.alpha, r, nchunks);
+                           (int)nx, (int)ny, (int)nz, (int)px, d, 1.0 / d, sc.eps, sc.alpha, r, nchunks);
     else
         hipLaunchKernelGGL((k_adv_sweep<AXIS, WENO, P, false>), dim3(blocks), dim3(256), 0, s, u,
-                           Lu, (int)nx, (int)ny, (int)nz, (int)px, d, sc.eps, sc.alpha, r, nchunks);
+                           Lu, (int)nx, (int)ny, (int)nz, (int)px, d, 1.0 / d, sc.eps, sc.alpha, r, nchunks);
 }
 
 // Arrays of px*ny*nz elements (row pitch px >= nx) addressable with 32-bit offsets.
@@ -283,6 +330,10 @@ void diff13(hipStream_t s, unsigned Nx, unsigned Ny, unsigned Nz, unsigned px, d
         return;
     }
     const DiffCoef c = diff_coef();
+    if (1ull * px * Ny < (1ull << 29)) {  // the z-march kernel's 32-bit plane offsets
+        diff13_b5_launch(s, Nx, Ny, Nz, px, q, Lq, c.kx, c.ky, c.kz);
+        return;
+    }
     const dim3 grid((Nx - 6 + 63) / 64, (Ny - 6 + 3) / 4, Nz - 5);
     hipLaunchKernelGGL(k_diff13_b5, grid, dim3(256), 0, s, q, Lq, (int)Nx, (int)Ny, (int)Nz, (int)px,
                        c.kx, c.ky, c.kz);

@@ -166,23 +166,35 @@ __global__ __launch_bounds__(256) void k_heat7_fe(const double *__restrict__ u, 
     }
 }
 
+// What k_heat13_rk writes: an SSP-RK3 stage of the heat step, the 13-point
+// operator itself (Call_Lu3d), or the operator added to Lu over the Burgers
+// viscous region (Call_Diff_).
+enum : int { kHeatRk = 0, kHeatLu = 1, kHeatDiffB5 = 2 };
+
 // One SSP-RK3 stage of the reference heat step fused: the 13-point Laplacian
 // of Compute_Laplace3d (Diffusion3d_baselineCode/kernels.cu:94-97, with the
 // CopyToConstantMemory coefficients) and Compute_RK (kernels.cu:127-140) on
 // the interior 2..n-3 of every axis, Lu kept in a register.  Loads run D
 // planes ahead of their first use: planes of the own columns sit in an
 // 8-slot register ring (k-2..k+2 in use), the halo and un in 4-slot rings.
-template <int TY, int D, bool LU = false>
+// MODE kHeatLu stores Lu instead.  MODE kHeatDiffB5 is Compute_Laplace_Async
+// (SingleGPU/Burgers3d_WENO5/kernels.cu:444-494): nodes 3..n-4 in x and y,
+// planes 3..nz-3, each axis's 4th-order difference summed from the low side
+// (-u_{-2} + 16 u_{-1} ... instead of from the high side), added to Lu,
+// which the un ring then reads (un = uout = Lu).
+template <int TY, int D, int MODE = kHeatRk>
 __global__ __launch_bounds__(256) void k_heat13_rk(const double *__restrict__ uin, const double *un, double *uout,
                                                    int nx, int ny, int nz, int px, int ntx, int nty, int kchunk,
                                                    double kx, double ky, double kz, double dt, int stage) {
     static_assert(D >= 1 && D <= 3, "prefetch distance");
+    constexpr bool LU = MODE == kHeatLu, B5 = MODE == kHeatDiffB5;
+    constexpr int LO = B5 ? 3 : 2;  // first interior node of x and y (and z)
     constexpr int RPT = TY / 4, W = kHeatTX + 4;
     __shared__ double sp[2][TY + 4][W];
     const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
     const int tix = tile % ntx, tiy = (tile / ntx) % nty, tiz = tile / (ntx * nty);
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int i0 = 2 + tix * kHeatTX, j0 = 2 + tiy * TY, z0 = 2 + tiz * kchunk, z1 = min(z0 + kchunk, nz - 2);
+    const int i0 = LO + tix * kHeatTX, j0 = LO + tiy * TY, z0 = LO + tiz * kchunk, z1 = min(z0 + kchunk, nz - 2);
     const int i = i0 + lane;
     const size_t xy = static_cast<size_t>(px) * ny;
     const unsigned pb = static_cast<unsigned>(xy * 8);
@@ -191,13 +203,14 @@ __global__ __launch_bounds__(256) void k_heat13_rk(const double *__restrict__ ui
     for (int r = 0; r < RPT; r++) {
         const int j = j0 + w + 4 * r;
         lo[r] = 8u * static_cast<unsigned>(min(i, nx - 1) + px * min(j, ny - 1));
-        so[r] = i < nx - 2 && j < ny - 2 ? lo[r] : pb;
+        so[r] = i < nx - LO && j < ny - LO ? lo[r] : pb;
     }
     HaloSlot hs[2];
     heat_halo<2, TY>(hs, w, lane, i0, j0, nx, ny, px);
     const unsigned ho0 = hs[0].row >= 0 ? hs[0].off : lo[0], ho1 = hs[1].row >= 0 ? hs[1].off : lo[0];
-    // un is read only by stages 2 and 3; stage 1 repeats uin loads instead
-    const double *ur = stage == 1 ? uin : un;
+    // un is read only by stages 2 and 3 (and as Lu by kHeatDiffB5); stage 1
+    // repeats uin loads instead
+    const double *ur = stage == 1 && !B5 ? uin : un;
     // own plane p in slot (p - z0 + 2) & 7, halo and un plane p in (p - z0) & 3;
     // planes past the chunk's last read are clamped to it
     const int zq = z1 + 1, zh = z1 - 1;
@@ -243,12 +256,21 @@ __global__ __launch_bounds__(256) void k_heat13_rk(const double *__restrict__ ui
         for (int r = 0; r < RPT; r++) {
             const int row = w + 4 * r + 2, x = lane + 2;
             const double c = q[r][(s + 2) & 7];
-            const double X = d4(P[row][x + 2], P[row][x + 1], c, P[row][x - 1], P[row][x - 2]);
-            const double Y = d4(P[row + 2][x], P[row + 1][x], c, P[row - 1][x], P[row - 2][x]);
-            const double Z = d4(q[r][(s + 4) & 7], q[r][(s + 3) & 7], c, q[r][(s + 1) & 7], q[r][s & 7]);
+            double X, Y, Z;
+            if (B5) {
+                X = d4(P[row][x - 2], P[row][x - 1], c, P[row][x + 1], P[row][x + 2]);
+                Y = d4(P[row - 2][x], P[row - 1][x], c, P[row + 1][x], P[row + 2][x]);
+                Z = d4(q[r][s & 7], q[r][(s + 1) & 7], c, q[r][(s + 3) & 7], q[r][(s + 4) & 7]);
+            } else {
+                X = d4(P[row][x + 2], P[row][x + 1], c, P[row][x - 1], P[row][x - 2]);
+                Y = d4(P[row + 2][x], P[row + 1][x], c, P[row - 1][x], P[row - 2][x]);
+                Z = d4(q[r][(s + 4) & 7], q[r][(s + 3) & 7], c, q[r][(s + 1) & 7], q[r][s & 7]);
+            }
             const double Lu = (kx * X + ky * Y) + kz * Z;
             if (LU) {
                 pv.store(so[r], Lu);
+            } else if (B5) {
+                pv.store(so[r], o[r][s & 3] + Lu);
             } else {
                 const double uo = stage == 1 ? c : o[r][s & 3];
                 pv.store(so[r], rk_stage(stage, dt, c, uo, Lu));
@@ -373,9 +395,22 @@ int heat13_lu_launch(hipStream_t s, unsigned nx, unsigned ny, unsigned nz, unsig
                      double *Lu) {
     const HeatGrid g = heat13_grid(nx - 4, ny - 4, nz - 4);
     const DiffCoef c = diff_coef();
-    const auto kern = k_heat13_rk<kHeat13TY, kHeat13D, true>;
+    const auto kern = k_heat13_rk<kHeat13TY, kHeat13D, kHeatLu>;
     hipLaunchKernelGGL(kern, dim3(g.blocks), dim3(256), 0, s, u, u, Lu, (int)nx, (int)ny, (int)nz, (int)px, g.ntx,
                        g.nty, g.kc, c.kx, c.ky, c.kz, 0.0, 1);
+    return check_launch();
+}
+
+// Compute_Laplace_Async (SingleGPU/Burgers3d_WENO5/kernels.cu:444-494,
+// Call_Diff_): Lu += the 13-point operator on nodes 3..n-4 in x and y and
+// planes 3..nz-3, with the same z-march; row pitch px, planes of
+// px*ny < 2^29 values, nx, ny >= 7, nz >= 6.
+int diff13_b5_launch(hipStream_t s, unsigned nx, unsigned ny, unsigned nz, unsigned px, const double *u, double *Lu,
+                     double kx, double ky, double kz) {
+    const HeatGrid g = heat13_grid(nx - 6, ny - 6, nz - 5);
+    const auto kern = k_heat13_rk<kHeat13TY, kHeat13D, kHeatDiffB5>;
+    hipLaunchKernelGGL(kern, dim3(g.blocks), dim3(256), 0, s, u, Lu, Lu, (int)nx, (int)ny, (int)nz, (int)px, g.ntx,
+                       g.nty, g.kc, kx, ky, kz, 0.0, 1);
     return check_launch();
 }
 

@@ -27,6 +27,8 @@ DiffCoef diff_coef();
 // Call_Lu3d's operator on the heat stage kernel (heat.hip).
 int heat13_lu_launch(hipStream_t s, unsigned nx, unsigned ny, unsigned nz, unsigned px, const double *u,
                      double *Lu);
+int diff13_b5_launch(hipStream_t s, unsigned nx, unsigned ny, unsigned nz, unsigned px, const double *u, double *Lu,
+                     double kx, double ky, double kz);
 
 // ---------------------------------------------------------------------------
 // Device: WENO5 arithmetic.
@@ -207,57 +209,6 @@ __device__ __forceinline__ double weno_face2(double v0, double v1, double v2, do
     return weno_face_s<WENO, P, FD>(v0, v1, v2, v3, v4, w0, w1, w2, w3, w4, sterm(v0, v1, v2),
                                 sterm(v1, v2, v3), sterm(v2, v3, v4), sterm(w0, w1, w2),
                                 sterm(w1, w2, w3), sterm(w2, w3, w4), eps);
-}
-
-// Straight restatement of the same face (kept for the drop-in sweeps).
-template <int WENO, int P>
-__device__ __forceinline__ double weno_face(double v0, double v1, double v2, double v3, double v4,
-                                            double w0, double w1, double w2, double w3, double w4,
-                                            double eps) {
-    double hp, hm;
-    {
-        const double B0 = smooth0(v0, v1, v2), B1 = smooth1(v1, v2, v3), B2 = smooth2(v2, v3, v4);
-        const double p0 = p2fma(2.0, v0, -(7.0 * v1)) + 11.0 * v2;
-        const double p1 = p2fma(2.0, v3, -v1 + 5.0 * v2);
-        const double p2 = p2fma(2.0, v2, 5.0 * v3) - v4;
-        if (WENO == kWenoZ) {
-            const double tau5 = fabs(B0 - B2);
-            const double a0 = zweight<P>(kD0N, tau5, B0, eps);
-            const double a1 = zweight<P>(kD1N, tau5, B1, eps);
-            const double a2 = zweight<P>(kD2N, tau5, B2, eps);
-            const double sum = (a0 + a1) + a2;
-            hp = ((a0 * p0 + a1 * p1) + a2 * p2) / (6.0 * sum);
-        } else {
-            const double a0 = kD0N / ((eps + B0) * (eps + B0));
-            const double a1 = kD1N / ((eps + B1) * (eps + B1));
-            const double a2 = kD2N / ((eps + B2) * (eps + B2));
-            const double sum = (a0 + a1) + a2;
-            const double o0 = a0 / sum, o1 = a1 / sum, o2 = a2 / sum;
-            hp = ((o0 * p0 + o1 * p1) + o2 * p2) / 6.0;
-        }
-    }
-    {
-        const double B0 = smooth0(w0, w1, w2), B1 = smooth1(w1, w2, w3), B2 = smooth2(w2, w3, w4);
-        const double m0 = p2fma(2.0, w2, -w0 + 5.0 * w1);
-        const double m1 = p2fma(2.0, w1, 5.0 * w2) - w3;
-        const double m2 = p2fma(2.0, w4, 11.0 * w2 - 7.0 * w3);
-        if (WENO == kWenoZ) {
-            const double tau5 = fabs(B0 - B2);
-            const double a0 = zweight<P>(kD0P, tau5, B0, eps);
-            const double a1 = zweight<P>(kD1P, tau5, B1, eps);
-            const double a2 = zweight<P>(kD2P, tau5, B2, eps);
-            const double sum = (a0 + a1) + a2;
-            hm = ((a0 * m0 + a1 * m1) + a2 * m2) / (6.0 * sum);
-        } else {
-            const double a0 = kD0P / ((eps + B0) * (eps + B0));
-            const double a1 = kD1P / ((eps + B1) * (eps + B1));
-            const double a2 = kD2P / ((eps + B2) * (eps + B2));
-            const double sum = (a0 + a1) + a2;
-            const double o0 = a0 / sum, o1 = a1 / sum, o2 = a2 / sum;
-            hm = ((o0 * m0 + o1 * m1) + o2 * m2) / 6.0;
-        }
-    }
-    return hp + hm;
 }
 
 // Lax-Friedrichs split of f(u) = u^2/2 (kernels.cu:37-40, 224-238):

@@ -54,7 +54,8 @@ def test_adv_sweeps(hipdev, shape, variant):
     assert np.array_equal(dLu.download(), ref)
 
 
-@pytest.mark.parametrize("shape", SHAPES)
+# (150, 75, 70): several 64 x 32 column tiles and z chunks of the z-march kernel
+@pytest.mark.parametrize("shape", SHAPES + [(150, 75, 70)])
 def test_diff_rk(hipdev, shape):
     nx, ny, nz = shape
     kx, ky, kz = 0.3, 0.2, 0.1
