@@ -316,6 +316,8 @@ int enqueue_step_p(int precision, hipStream_t stream, const mgad_scheme &sc, uns
                                 static_cast<double *>(w1), static_cast<double *>(w2), ctrl);
 }
 
+// A captured step is replayed only while its launch shape is current: the
+// tile width and z chunk (mgad_burgers_set_launch, autotune) are part of the key.
 struct GraphKey {
     hipStream_t stream;
     int precision, weno, p;
@@ -323,10 +325,11 @@ struct GraphKey {
     unsigned nx, ny, nz;
     double dx, dy, dz;
     void *u, *w1, *w2, *ctrl;
+    int tile, kchunk;
     bool operator<(const GraphKey &o) const {
-        return std::tie(stream, precision, weno, p, eps, nx, ny, nz, dx, dy, dz, u, w1, w2, ctrl) <
+        return std::tie(stream, precision, weno, p, eps, nx, ny, nz, dx, dy, dz, u, w1, w2, ctrl, tile, kchunk) <
                std::tie(o.stream, o.precision, o.weno, o.p, o.eps, o.nx, o.ny, o.nz, o.dx, o.dy, o.dz,
-                        o.u, o.w1, o.w2, o.ctrl);
+                        o.u, o.w1, o.w2, o.ctrl, o.tile, o.kchunk);
     }
 };
 
@@ -350,8 +353,9 @@ int run_steps(int precision, hipStream_t stream, const mgad_scheme *scheme, unsi
         }
         return MGAD_OK;
     }
+    const LaunchCfg cfg = launch_cfg(precision, *scheme, nx, ny, nz);
     const GraphKey key{stream, precision, scheme->weno, scheme->p, scheme->eps, nx, ny, nz, dx, dy, dz,
-                       u,      w1,        w2,           ctrl};
+                       u,      w1,        w2,           ctrl,       cfg.tile,  cfg.kchunk};
     auto &cache = graph_cache();
     auto it = cache.find(key);
     hipGraphExec_t exec = nullptr;

@@ -219,10 +219,14 @@ extern "C" int mgad_burgers_slab_steps(hipStream_t stream, const mgad_scheme *sc
         }
         return MGAD_OK;
     }
+    // the launch shape taken from the unsplit grid's table is part of the key,
+    // so a changed table entry is captured afresh
+    const LaunchCfg cfg = launch_cfg(MGAD_F64, *scheme, nx, ny, nzs * nslab);
     using Key = std::tuple<hipStream_t, int, int, double, unsigned, unsigned, unsigned, unsigned, double, double,
-                           double, double *, double *, double *, void *>;
+                           double, double *, double *, double *, void *, int, int>;
     static std::map<Key, hipGraphExec_t> cache;
-    const Key key{stream, scheme->weno, scheme->p, scheme->eps, nx, ny, nzs, nslab, dx, dy, dz, u, w1, w2, ctrl};
+    const Key key{stream, scheme->weno, scheme->p, scheme->eps, nx, ny, nzs, nslab, dx, dy, dz, u, w1, w2, ctrl,
+                  cfg.tile, cfg.kchunk};
     hipGraphExec_t exec = nullptr;
     auto it = cache.find(key);
     if (it == cache.end()) {
