@@ -231,6 +231,7 @@ CfgKey cfg_key(int precision, const mgad_scheme &sc, unsigned nx, unsigned ny, u
 }  // namespace
 
 LaunchCfg launch_cfg(int precision, const mgad_scheme &sc, unsigned nx, unsigned ny, unsigned nz) {
+    std::lock_guard<std::recursive_mutex> lock(host_mutex());
     static const int env_tile = env_int("MGAD_TILE"), env_kc = env_int("MGAD_KCHUNK");
     LaunchCfg c{0, 0};
     auto it = cfg_table().find(cfg_key(precision, sc, nx, ny, nz));
@@ -241,6 +242,7 @@ LaunchCfg launch_cfg(int precision, const mgad_scheme &sc, unsigned nx, unsigned
 }
 
 void set_launch_cfg(int precision, const mgad_scheme &sc, unsigned nx, unsigned ny, unsigned nz, LaunchCfg c) {
+    std::lock_guard<std::recursive_mutex> lock(host_mutex());
     cfg_table()[cfg_key(precision, sc, nx, ny, nz)] = c;
 }
 
@@ -353,6 +355,7 @@ int run_steps(int precision, hipStream_t stream, const mgad_scheme *scheme, unsi
         }
         return MGAD_OK;
     }
+    std::lock_guard<std::recursive_mutex> lock(host_mutex());
     const LaunchCfg cfg = launch_cfg(precision, *scheme, nx, ny, nz);
     const GraphKey key{stream, precision, scheme->weno, scheme->p, scheme->eps, nx, ny, nz, dx, dy, dz,
                        u,      w1,        w2,           ctrl,       cfg.tile,  cfg.kchunk};

@@ -358,6 +358,7 @@ extern "C" int mgad_burgers_ref_steps(hipStream_t stream, const mgad_scheme *sch
     int st = check_hip(hipMemcpyAsync(w1, u, bytes, hipMemcpyDeviceToDevice, stream));
     if (!st) st = check_hip(hipMemcpyAsync(w2, u, bytes, hipMemcpyDeviceToDevice, stream));
     if (st || nsteps == 0) return st;
+    std::lock_guard<std::recursive_mutex> lock(host_mutex());
     RefLaunch L = ref_launch(*scheme, viscous != 0, nx, ny, nz, dx, dy, dz);
     L.g.amax = ref_amax_slots(stream);
     if (!L.g.amax) return MGAD_ERR_HIP;

@@ -443,6 +443,7 @@ extern "C" int mgad_heat7_fe_steps(hipStream_t stream, unsigned int nx, unsigned
     }
     if (!use_graph || stream == nullptr || nsteps == 0)
         return enqueue_heat(stream, nx, ny, nz, kx, ky, kz, dt, u, w, nsteps);
+    std::lock_guard<std::recursive_mutex> lock(host_mutex());
     static std::map<HeatKey, hipGraphExec_t> cache;
     const HeatKey key{stream, nx, ny, nz, kx, ky, kz, dt, u, w, nsteps};
     auto it = cache.find(key);
@@ -491,6 +492,7 @@ extern "C" int mgad_heat13_rk3_steps(hipStream_t stream, unsigned int nx, unsign
         for (int s = 0; s < nsteps && !st; s++) st = enqueue_heat13(stream, nx, ny, nz, c, dt, u, w1, w2);
         return st;
     }
+    std::lock_guard<std::recursive_mutex> lock(host_mutex());
     using Key = std::tuple<hipStream_t, unsigned, unsigned, unsigned, double, double, double, double, double *,
                            double *, double *>;
     static std::map<Key, hipGraphExec_t> cache;

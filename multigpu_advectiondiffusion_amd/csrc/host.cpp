@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
+#include <mutex>
 #include <thread>
 #include <vector>
 
@@ -20,6 +21,11 @@ double g_kx = 0.0, g_ky = 0.0, g_kz = 0.0;
 }  // namespace
 
 struct DiffCoef { double kx, ky, kz; };
+
+std::recursive_mutex &host_mutex() {
+    static std::recursive_mutex m;
+    return m;
+}
 
 void set_error(int code) {
     int expected = 0;

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <mutex>
 
 #include "../../include/mgad.h"
 
@@ -20,6 +21,12 @@ namespace mgad {
 void set_error(int code);
 int check_hip(hipError_t e);           // records MGAD_ERR_HIP, returns status
 int check_launch();                    // hipGetLastError after a launch
+
+// One lock for the library's host-side tables and caches (launch shapes,
+// captured step graphs, per-stream slots), so entry points may be called
+// from several host threads.  Recursive: an entry point that holds it reads
+// the launch table, which takes it again.
+std::recursive_mutex &host_mutex();
 
 // Diffusion coefficients of CopyToConstantMemory, passed to kernels by value.
 struct DiffCoef { double kx, ky, kz; };

@@ -219,6 +219,7 @@ extern "C" int mgad_burgers_slab_steps(hipStream_t stream, const mgad_scheme *sc
         }
         return MGAD_OK;
     }
+    std::lock_guard<std::recursive_mutex> lock(host_mutex());
     // the launch shape taken from the unsplit grid's table is part of the key,
     // so a changed table entry is captured afresh
     const LaunchCfg cfg = launch_cfg(MGAD_F64, *scheme, nx, ny, nzs * nslab);

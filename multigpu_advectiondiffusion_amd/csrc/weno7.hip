@@ -422,6 +422,7 @@ extern "C" int mgad_weno7_steps(hipStream_t stream, unsigned int nx, unsigned in
         }
         return MGAD_OK;
     }
+    std::lock_guard<std::recursive_mutex> lock(host_mutex());
     using Key = std::tuple<hipStream_t, unsigned, unsigned, unsigned, double, double, double, double *, double *,
                            double *, double *, void *>;
     static std::map<Key, hipGraphExec_t> cache;

@@ -92,6 +92,35 @@ def test_graph_replays_match_eager(hipdev):
     assert a.state().steps == 12
 
 
+def test_host_threads_share_the_library(hipdev):
+    """Four host threads, each with its own stream and grid size, capture and
+    replay step graphs at the same time (the graph caches and launch table are
+    shared); every result equals the same run done alone."""
+    import threading
+
+    sizes = (32, 48, 32, 64)
+    u0s = [solvers.ic_fourier(n, seed=20 + i) for i, n in enumerate(sizes)]
+
+    def run(n, u0, out, i):
+        s = solvers.PeriodicBurgers(n, scheme="Z2")
+        s.set_initial(u0, tend=10.0)
+        for _ in range(5):
+            s.steps(4, graph=True)
+        out[i] = s.solution()
+
+    alone = [None] * len(sizes)
+    for i, (n, u0) in enumerate(zip(sizes, u0s)):
+        run(n, u0, alone, i)
+    together = [None] * len(sizes)
+    threads = [threading.Thread(target=run, args=(n, u0, together, i)) for i, (n, u0) in enumerate(zip(sizes, u0s))]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join(timeout=120)
+    for a, b in zip(alone, together):
+        assert b is not None and np.array_equal(a, b)
+
+
 def test_no_op_after_tend(hipdev):
     n = 32
     s = solvers.PeriodicBurgers(n, scheme="JS")
