@@ -44,8 +44,8 @@ struct SweepRange {
 
 // Face flux with the 13/12 terms supplied; fd (the same for the whole wave)
 // selects the shortened weight divisions, whose results equal IEEE division
-// when every node read is at most 2^40 in magnitude and eps is in range
-// (fast_division_ok).
+// when every node read and a global LF alpha are at most 2^40 in magnitude
+// and eps is in range (fast_division_ok).
 template <int WENO, int P>
 __device__ __forceinline__ double face_fd(bool fd, double v0, double v1, double v2, double v3, double v4, double w0,
                                           double w1, double w2, double w3, double w4, double Sp0, double Sp1,
@@ -97,7 +97,8 @@ __global__ __launch_bounds__(256) void k_adv_sweep(const double *__restrict__ u,
     const int i1 = min(i0 + kChunk, r.l1);
     if (i0 >= i1) return;
     const bool low_face = i0 > r.l0 || !r.zero_first;
-    const bool eps_ok = fast_division_ok<WENO>(0.0, eps);
+    // a global LF alpha enters every split flux, so it is bounded like |u|
+    const bool eps_ok = fast_division_ok<WENO>(LOCAL ? 0.0 : alpha, eps);
 
     bool ok = true;  // every node read so far is at most 2^40 in magnitude
     auto take = [&](double v, double &fp, double &fm) {
@@ -191,7 +192,7 @@ __global__ __launch_bounds__(256) void k_adv_x_rows(const double *__restrict__ u
     S[r][lane] = split(u[base + min(max(i0 - 3 + lane, 0), nx - 1)]);
     if (lane < 5) S[r][64 + lane] = split(u[base + min(i0 + 61 + lane, nx - 1)]);
     // one wave per row: the row's fast-division test is wave-uniform
-    const bool fd = fast_division_ok<WENO>(0.0, eps) && __builtin_amdgcn_ballot_w64(!ok) == 0;
+    const bool fd = fast_division_ok<WENO>(LOCAL ? 0.0 : alpha, eps) && __builtin_amdgcn_ballot_w64(!ok) == 0;
     __syncthreads();
     {
         const double2 a0 = S[r][lane], a1 = S[r][lane + 1], a2 = S[r][lane + 2], a3 = S[r][lane + 3],

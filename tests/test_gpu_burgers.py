@@ -59,6 +59,43 @@ def test_stages_bitwise(hipdev, scheme, shape):
     assert st.amax == O.maxabs(u3)
 
 
+# The stage kernels shorten the weight divisions only when max|u| <= 2^40 and
+# eps is in range (2^-300..1 for Z, 2^-190..1 for JS); these cases fall
+# outside and take the IEEE sequence, which must match the oracle as well.
+@pytest.mark.parametrize("case", ["z_eps_1e-95", "js_eps_1e-100", "z_u_times_2^41"])
+def test_stages_ieee_division_path_bitwise(hipdev, case):
+    from multigpu_advectiondiffusion_amd import _capi
+
+    shape = (32, 48, 20)
+    scheme = "JS" if case.startswith("js") else "Z2"
+    w, p, eps = solvers.SCHEMES[scheme]
+    u0 = solvers.ic_fourier(shape, seed=9)
+    if case == "z_eps_1e-95":
+        eps = 1e-95
+    elif case == "js_eps_1e-100":
+        eps = 1e-100
+    else:
+        u0 = u0 * 2.0 ** 41
+    orc = dict(weno=O.WENO_JS if scheme == "JS" else O.WENO_Z, p=p, eps=eps, lf=O.LF_GLOBAL)
+    s = solvers.PeriodicBurgers(shape, scheme=scheme)
+    s.scheme = _capi.Scheme(w, p, eps)
+    s.set_initial(u0, tend=10.0)
+    d = (s.dx, s.dy, s.dz)
+    u_prev, stages = u0, ((1, s.u, s.w1), (2, s.w1, s.w2), (3, s.w2, s.u))
+    for stage, src, dst in stages:
+        s.stage(stage, src, s.u, dst)
+        if stage == 1:
+            dt = s.state().dt
+        Lu = O.rhs_periodic(u_prev, d, **orc, alpha=O.maxabs(u_prev))
+        ref = u_prev.copy()
+        O.rk_stage_all(ref, u0, Lu, stage, dt)
+        s.stream.synchronize()
+        got = dst.download()
+        assert np.isfinite(ref).all()
+        assert np.array_equal(got, ref), (stage, np.argwhere(got != ref)[:3])
+        u_prev = ref
+
+
 @pytest.mark.parametrize("scheme", ["Z2", "JS"])
 @pytest.mark.parametrize("graph", [False, True])
 def test_run_to_half_bitwise(hipdev, scheme, graph):

@@ -30,15 +30,24 @@ def reset_scheme():
     R.set_adv_scheme()
 
 
+# The sweeps shorten the weight divisions per wave when every node it reads
+# (and a global alpha) is at most 2^40 and eps is in range: "big_alpha",
+# "tiny_eps" and "big_block" (a block of nodes above 2^40, so some waves of
+# each sweep take the IEEE sequence and others do not) exercise the other path.
 @pytest.mark.parametrize("shape", SHAPES)
-@pytest.mark.parametrize("variant", ["reference", "js_global", "z2_local"])
+@pytest.mark.parametrize("variant", ["reference", "js_global", "z2_local", "big_alpha", "tiny_eps", "big_block"])
 def test_adv_sweeps(hipdev, shape, variant):
     nx, ny, nz = shape
     kw = {"reference": dict(O.REFERENCE),
           "js_global": dict(weno=O.WENO_JS, p=2, eps=1e-6, lf=O.LF_GLOBAL, alpha=2.5),
-          "z2_local": dict(weno=O.WENO_Z, p=2, eps=1e-40, lf=O.LF_LOCAL)}[variant]
+          "z2_local": dict(weno=O.WENO_Z, p=2, eps=1e-40, lf=O.LF_LOCAL),
+          "big_alpha": dict(weno=O.WENO_Z, p=2, eps=1e-40, lf=O.LF_GLOBAL, alpha=2.0 ** 45),
+          "tiny_eps": dict(weno=O.WENO_Z, p=1, eps=1e-95, lf=O.LF_LOCAL),
+          "big_block": dict(O.REFERENCE)}[variant]
     R.set_adv_scheme(kw["weno"], kw["p"], kw["eps"], kw["lf"], kw.get("alpha", 0.0))
     u = field(shape, 1)
+    if variant == "big_block":
+        u[: nz // 2, : ny // 2, : nx // 2] *= 2.0 ** 42
     Lu0 = field(shape, 2)
     du, dLu = DeviceArray.from_host(u), DeviceArray.from_host(Lu0)
     ref = Lu0.copy()
