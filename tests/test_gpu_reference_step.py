@@ -51,6 +51,29 @@ def test_fused_reference_step_bitwise(hipdev, shape, scheme, viscous, graph):
     assert np.array_equal(got, ref), np.argwhere(got != ref)[:5]
 
 
+@pytest.mark.parametrize("graph", [False, True])
+def test_fused_reference_step_ieee_division_path(hipdev, graph):
+    """A block of nodes above 2^40 in magnitude: the stages take the IEEE
+    weight divisions (the fused kernel's fast-division test is on the stage
+    input's max|u|), still bit for bit with the oracle.  The block lies in
+    the interior, away from the frozen ghost layers, so every stage updates it."""
+    nx, ny, nz = 40, 32, 24
+    dx = dy = dz = 2.0 / 39
+    kc = (1e-5 / (12 * dx * dx),) * 3
+    R.CopyToConstantMemory(*kc)
+    u0 = fileio.init3d_burgers(3, (nx, ny, nz), dx, dy, dz)
+    blk = u0[8:12, 10:14, 12:20]
+    u0[8:12, 10:14, 12:20] = 2.0 ** 41 * (1.0 + 0.1 * np.random.default_rng(3).standard_normal(blk.shape))
+    dt = 1e-16
+    s = solvers.ReferenceBurgers((nx, ny, nz), dx, dy, dz, scheme="Z1", viscous=True)
+    s.set_initial(u0)
+    s.steps(2, dt, graph=graph)
+    ref = oracle_steps(u0, 2, dx, dy, dz, dt, ORACLE_KW["Z1"], True, kc)
+    got = s.solution()
+    assert np.isfinite(ref).all()
+    assert np.array_equal(got, ref), np.argwhere(got != ref)[:5]
+
+
 def test_fused_reference_step_matches_dropins_on_gpu(hipdev):
     """Same u as the GPU drop-in sequence itself (reference_api.burgers_reference_step)."""
     from multigpu_advectiondiffusion_amd.hip import DeviceArray
