@@ -4,6 +4,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "mgad_common.hpp"
 
 namespace mgad {
@@ -150,9 +152,12 @@ __device__ __forceinline__ void stage_tile(const T *__restrict__ uin, const T *u
         hcol = static_cast<unsigned>(x0 + c) + static_cast<unsigned>(nx) * wrapi(y0 + hl_r - 3, ny);
     }
 
-    // z register queues: qp/qm[q] hold the split fluxes of plane k-3+q,
-    // sp/sn[q] the 13/12 terms centred on plane k-1+q, qu[q] u_in of plane k+q.
-    double qp[7], qm[7], sp[4], sn[4], qu[4];
+    // z register rings (see the march below): qp/qm hold the split fluxes of
+    // planes k-3..k+3, plane p in slot (p - z0 + 3) & 7; sp/sn the 13/12 terms
+    // centred on planes k-1..k+2, centre c in slot (c - z0 + 1) & 3; qu u_in of
+    // planes k..k+3, plane p in slot (p - z0) & 3.  At k = z0 these are the
+    // slots q of plane k-3+q, centre k-1+q and plane k+q.
+    double qp[8], qm[8], sp[4], sn[4], qu[4];
 #pragma unroll
     for (int q = 0; q < 6; q++) {
         const double v = ld(uin, col + plane * zw(z0 - 3 + q));
@@ -181,95 +186,111 @@ __device__ __forceinline__ void stage_tile(const T *__restrict__ uin, const T *u
     double un_next = (stage == 1) ? 0.0 : ld(un_ptr, plane * zi(z0));
     unsigned long long vmax = 0ull;
 
-    for (int k = z0; k < z1; k++) {
-        const double own = own_next, halo = halo_next, unv = un_next;
-        if (k + 1 < z1) {
-            kn = (!ZG && kn + 1 == static_cast<unsigned>(nz)) ? 0u : kn + 1;
-            own_next = ld(own_ptr, plane * kn);
-            if (has_halo) halo_next = ld(halo_ptr, plane * zi(k + 1));
-            if (stage != 1) un_next = ld(un_ptr, plane * zi(k + 1));
-        }
-        lf_split(own, alpha, qp[6], qm[6]);
-        qu[3] = own;
-        sp[3] = sterm(qp[4], qp[5], qp[6]);  // centre k+2
-        sn[3] = sterm(qm[4], qm[5], qm[6]);
-        sm.P[ty + 3][tx + 3] = make_double2(qp[3], qm[3]);
-        if (has_halo) {
-            double hp, hm;
-            lf_split(halo, alpha, hp, hm);
-            sm.P[hl_r][hl_c] = make_double2(hp, hm);
-        }
-        __syncthreads();
-
-        // z face k+1/2 from the register queue; x face x+1/2 and y face y+1/2 from LDS
-        const double fz = weno_face_s<WENO, P, FD>(qp[1], qp[2], qp[3], qp[4], qp[5], qm[2], qm[3], qm[4],
-                                               qm[5], qm[6], sp[0], sp[1], sp[2], sn[1], sn[2], sn[3],
-                                               eps);
-        {
-            const double2 a0 = sm.P[ty + 3][tx + 1], a1 = sm.P[ty + 3][tx + 2], a2 = sm.P[ty + 3][tx + 3],
-                          a3 = sm.P[ty + 3][tx + 4], a4 = sm.P[ty + 3][tx + 5], a5 = sm.P[ty + 3][tx + 6];
-            sm.FX[ty][tx + 1] = weno_face2<WENO, P, FD>(a0.x, a1.x, a2.x, a3.x, a4.x, a1.y, a2.y, a3.y,
-                                                    a4.y, a5.y, eps);
-        }
-        {
-            const double2 a0 = sm.P[ty + 1][tx + 3], a1 = sm.P[ty + 2][tx + 3], a2 = sm.P[ty + 3][tx + 3],
-                          a3 = sm.P[ty + 4][tx + 3], a4 = sm.P[ty + 5][tx + 3], a5 = sm.P[ty + 6][tx + 3];
-            sm.FY[ty + 1][tx] = weno_face2<WENO, P, FD>(a0.x, a1.x, a2.x, a3.x, a4.x, a1.y, a2.y, a3.y,
-                                                    a4.y, a5.y, eps);
-        }
-        // faces on the tile's low edges, by one wave that rotates with k
-        if (wave == k % NW && lane < TY + TX) {
-            double2 a0, a1, a2, a3, a4, a5;
-            if (lane < TY) {
-                const int r = lane + 3;
-                a0 = sm.P[r][0]; a1 = sm.P[r][1]; a2 = sm.P[r][2];
-                a3 = sm.P[r][3]; a4 = sm.P[r][4]; a5 = sm.P[r][5];
-            } else {
-                const int c = lane - TY + 3;
-                a0 = sm.P[0][c]; a1 = sm.P[1][c]; a2 = sm.P[2][c];
-                a3 = sm.P[3][c]; a4 = sm.P[4][c]; a5 = sm.P[5][c];
+    // The queues are rings over the z march, indexed by the plane's offset
+    // from z0 modulo 8 (split fluxes, planes k-3..k+3) or 4 (13/12 terms,
+    // centres k-1..k+2; u_in, planes k..k+3), and the march is unrolled by 8,
+    // so every index is a constant and no register moves shift the queues.
+    int kb = z0;
+    auto zstep = [&](auto S) {
+        constexpr int s = decltype(S)::value;
+        auto R8 = [](int q) { return (s + q) & 7; };
+        auto R4 = [](int q) { return (s + q) & 3; };
+        const int k = kb + s;
+            const double own = own_next, halo = halo_next, unv = un_next;
+            if (k + 1 < z1) {
+                kn = (!ZG && kn + 1 == static_cast<unsigned>(nz)) ? 0u : kn + 1;
+                own_next = ld(own_ptr, plane * kn);
+                if (has_halo) halo_next = ld(halo_ptr, plane * zi(k + 1));
+                if (stage != 1) un_next = ld(un_ptr, plane * zi(k + 1));
             }
-            const double f = weno_face2<WENO, P, FD>(a0.x, a1.x, a2.x, a3.x, a4.x, a1.y, a2.y, a3.y, a4.y,
-                                                 a5.y, eps);
-            if (lane < TY)
-                sm.FX[lane][0] = f;
+            lf_split(own, alpha, qp[R8(6)], qm[R8(6)]);
+            qu[R4(3)] = own;
+            sp[R4(3)] = sterm(qp[R8(4)], qp[R8(5)], qp[R8(6)]);  // centre k+2
+            sn[R4(3)] = sterm(qm[R8(4)], qm[R8(5)], qm[R8(6)]);
+            sm.P[ty + 3][tx + 3] = make_double2(qp[R8(3)], qm[R8(3)]);
+            if (has_halo) {
+                double hp, hm;
+                lf_split(halo, alpha, hp, hm);
+                sm.P[hl_r][hl_c] = make_double2(hp, hm);
+            }
+            __syncthreads();
+
+            // z face k+1/2 from the register queue; x face x+1/2 and y face y+1/2 from LDS
+            const double fz = weno_face_s<WENO, P, FD>(qp[R8(1)], qp[R8(2)], qp[R8(3)], qp[R8(4)], qp[R8(5)],
+                                                       qm[R8(2)], qm[R8(3)], qm[R8(4)], qm[R8(5)], qm[R8(6)],
+                                                       sp[R4(0)], sp[R4(1)], sp[R4(2)], sn[R4(1)], sn[R4(2)],
+                                                       sn[R4(3)], eps);
+            {
+                const double2 a0 = sm.P[ty + 3][tx + 1], a1 = sm.P[ty + 3][tx + 2], a2 = sm.P[ty + 3][tx + 3],
+                              a3 = sm.P[ty + 3][tx + 4], a4 = sm.P[ty + 3][tx + 5], a5 = sm.P[ty + 3][tx + 6];
+                sm.FX[ty][tx + 1] = weno_face2<WENO, P, FD>(a0.x, a1.x, a2.x, a3.x, a4.x, a1.y, a2.y, a3.y,
+                                                        a4.y, a5.y, eps);
+            }
+            {
+                const double2 a0 = sm.P[ty + 1][tx + 3], a1 = sm.P[ty + 2][tx + 3], a2 = sm.P[ty + 3][tx + 3],
+                              a3 = sm.P[ty + 4][tx + 3], a4 = sm.P[ty + 5][tx + 3], a5 = sm.P[ty + 6][tx + 3];
+                sm.FY[ty + 1][tx] = weno_face2<WENO, P, FD>(a0.x, a1.x, a2.x, a3.x, a4.x, a1.y, a2.y, a3.y,
+                                                        a4.y, a5.y, eps);
+            }
+            // faces on the tile's low edges, by one wave that rotates with k
+            if (wave == k % NW && lane < TY + TX) {
+                double2 a0, a1, a2, a3, a4, a5;
+                if (lane < TY) {
+                    const int r = lane + 3;
+                    a0 = sm.P[r][0]; a1 = sm.P[r][1]; a2 = sm.P[r][2];
+                    a3 = sm.P[r][3]; a4 = sm.P[r][4]; a5 = sm.P[r][5];
+                } else {
+                    const int c = lane - TY + 3;
+                    a0 = sm.P[0][c]; a1 = sm.P[1][c]; a2 = sm.P[2][c];
+                    a3 = sm.P[3][c]; a4 = sm.P[4][c]; a5 = sm.P[5][c];
+                }
+                const double f = weno_face2<WENO, P, FD>(a0.x, a1.x, a2.x, a3.x, a4.x, a1.y, a2.y, a3.y, a4.y,
+                                                     a5.y, eps);
+                if (lane < TY)
+                    sm.FX[lane][0] = f;
+                else
+                    sm.FY[0][lane - TY] = f;
+            }
+            __syncthreads();
+
+            double Lu = -div_by(sm.FX[ty][tx + 1] - sm.FX[ty][tx], g.dx, g.rdx);
+            Lu -= div_by(sm.FY[ty + 1][tx] - sm.FY[ty][tx], g.dy, g.rdy);
+            Lu -= div_by(fz - fz_prev, g.dz, g.rdz);
+            const double uo = (stage == 1) ? qu[R4(0)] : unv;
+            double unew;
+            if (stage == 1)
+                unew = uo + dt * Lu;
+            else if (stage == 2)
+                unew = 0.75 * uo + 0.25 * (qu[R4(0)] + dt * Lu);
             else
-                sm.FY[0][lane - TY] = f;
-        }
-        __syncthreads();
+                unew = div_by(p2fma(2.0, qu[R4(0)] + dt * Lu, uo), 3.0, 1.0 / 3.0);  // (uo + 2(..)) / 3
+            const T stored = static_cast<T>(unew);
+            out_ptr[plane * zi(k)] = stored;
+            if (ZG) {
+                if (k < 3) gh_lo[col + plane * static_cast<unsigned>(nz + 3 + k)] = stored;
+                if (k >= nz - 3) gh_hi[col + plane * static_cast<unsigned>(k - nz + 3)] = stored;
+            }
+            const unsigned long long b = abs_bits(static_cast<double>(stored));
+            vmax = b > vmax ? b : vmax;
 
-        double Lu = -div_by(sm.FX[ty][tx + 1] - sm.FX[ty][tx], g.dx, g.rdx);
-        Lu -= div_by(sm.FY[ty + 1][tx] - sm.FY[ty][tx], g.dy, g.rdy);
-        Lu -= div_by(fz - fz_prev, g.dz, g.rdz);
-        const double uo = (stage == 1) ? qu[0] : unv;
-        double unew;
-        if (stage == 1)
-            unew = uo + dt * Lu;
-        else if (stage == 2)
-            unew = 0.75 * uo + 0.25 * (qu[0] + dt * Lu);
-        else
-            unew = div_by(p2fma(2.0, qu[0] + dt * Lu, uo), 3.0, 1.0 / 3.0);  // (uo + 2(..)) / 3
-        const T stored = static_cast<T>(unew);
-        out_ptr[plane * zi(k)] = stored;
-        if (ZG) {
-            if (k < 3) gh_lo[col + plane * static_cast<unsigned>(nz + 3 + k)] = stored;
-            if (k >= nz - 3) gh_hi[col + plane * static_cast<unsigned>(k - nz + 3)] = stored;
-        }
-        const unsigned long long b = abs_bits(static_cast<double>(stored));
-        vmax = b > vmax ? b : vmax;
-
-        fz_prev = fz;
-#pragma unroll
-        for (int q = 0; q < 6; q++) {
-            qp[q] = qp[q + 1];
-            qm[q] = qm[q + 1];
-        }
-#pragma unroll
-        for (int q = 0; q < 3; q++) {
-            sp[q] = sp[q + 1];
-            sn[q] = sn[q + 1];
-            qu[q] = qu[q + 1];
-        }
+            fz_prev = fz;
+    };
+    for (; kb < z1; kb += 8) {
+        zstep(std::integral_constant<int, 0>{});
+        if (kb + 1 >= z1) break;
+        zstep(std::integral_constant<int, 1>{});
+        if (kb + 2 >= z1) break;
+        zstep(std::integral_constant<int, 2>{});
+        if (kb + 3 >= z1) break;
+        zstep(std::integral_constant<int, 3>{});
+        if (kb + 4 >= z1) break;
+        zstep(std::integral_constant<int, 4>{});
+        if (kb + 5 >= z1) break;
+        zstep(std::integral_constant<int, 5>{});
+        if (kb + 6 >= z1) break;
+        zstep(std::integral_constant<int, 6>{});
+        if (kb + 7 >= z1) break;
+        zstep(std::integral_constant<int, 7>{});
     }
 
     // ---- block max -> one atomic per block ----
