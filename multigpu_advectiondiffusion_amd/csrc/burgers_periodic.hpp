@@ -196,84 +196,84 @@ __device__ __forceinline__ void stage_tile(const T *__restrict__ uin, const T *u
         auto R8 = [](int q) { return (s + q) & 7; };
         auto R4 = [](int q) { return (s + q) & 3; };
         const int k = kb + s;
-            const double own = own_next, halo = halo_next, unv = un_next;
-            if (k + 1 < z1) {
-                kn = (!ZG && kn + 1 == static_cast<unsigned>(nz)) ? 0u : kn + 1;
-                own_next = ld(own_ptr, plane * kn);
-                if (has_halo) halo_next = ld(halo_ptr, plane * zi(k + 1));
-                if (stage != 1) un_next = ld(un_ptr, plane * zi(k + 1));
-            }
-            lf_split(own, alpha, qp[R8(6)], qm[R8(6)]);
-            qu[R4(3)] = own;
-            sp[R4(3)] = sterm(qp[R8(4)], qp[R8(5)], qp[R8(6)]);  // centre k+2
-            sn[R4(3)] = sterm(qm[R8(4)], qm[R8(5)], qm[R8(6)]);
-            sm.P[ty + 3][tx + 3] = make_double2(qp[R8(3)], qm[R8(3)]);
-            if (has_halo) {
-                double hp, hm;
-                lf_split(halo, alpha, hp, hm);
-                sm.P[hl_r][hl_c] = make_double2(hp, hm);
-            }
-            __syncthreads();
+        const double own = own_next, halo = halo_next, unv = un_next;
+        if (k + 1 < z1) {
+            kn = (!ZG && kn + 1 == static_cast<unsigned>(nz)) ? 0u : kn + 1;
+            own_next = ld(own_ptr, plane * kn);
+            if (has_halo) halo_next = ld(halo_ptr, plane * zi(k + 1));
+            if (stage != 1) un_next = ld(un_ptr, plane * zi(k + 1));
+        }
+        lf_split(own, alpha, qp[R8(6)], qm[R8(6)]);
+        qu[R4(3)] = own;
+        sp[R4(3)] = sterm(qp[R8(4)], qp[R8(5)], qp[R8(6)]);  // centre k+2
+        sn[R4(3)] = sterm(qm[R8(4)], qm[R8(5)], qm[R8(6)]);
+        sm.P[ty + 3][tx + 3] = make_double2(qp[R8(3)], qm[R8(3)]);
+        if (has_halo) {
+            double hp, hm;
+            lf_split(halo, alpha, hp, hm);
+            sm.P[hl_r][hl_c] = make_double2(hp, hm);
+        }
+        __syncthreads();
 
-            // z face k+1/2 from the register queue; x face x+1/2 and y face y+1/2 from LDS
-            const double fz = weno_face_s<WENO, P, FD>(qp[R8(1)], qp[R8(2)], qp[R8(3)], qp[R8(4)], qp[R8(5)],
-                                                       qm[R8(2)], qm[R8(3)], qm[R8(4)], qm[R8(5)], qm[R8(6)],
-                                                       sp[R4(0)], sp[R4(1)], sp[R4(2)], sn[R4(1)], sn[R4(2)],
-                                                       sn[R4(3)], eps);
-            {
-                const double2 a0 = sm.P[ty + 3][tx + 1], a1 = sm.P[ty + 3][tx + 2], a2 = sm.P[ty + 3][tx + 3],
-                              a3 = sm.P[ty + 3][tx + 4], a4 = sm.P[ty + 3][tx + 5], a5 = sm.P[ty + 3][tx + 6];
-                sm.FX[ty][tx + 1] = weno_face2<WENO, P, FD>(a0.x, a1.x, a2.x, a3.x, a4.x, a1.y, a2.y, a3.y,
-                                                        a4.y, a5.y, eps);
+        // z face k+1/2 from the register queue; x face x+1/2 and y face y+1/2 from LDS
+        const double fz = weno_face_s<WENO, P, FD>(qp[R8(1)], qp[R8(2)], qp[R8(3)], qp[R8(4)], qp[R8(5)],
+                                                   qm[R8(2)], qm[R8(3)], qm[R8(4)], qm[R8(5)], qm[R8(6)],
+                                                   sp[R4(0)], sp[R4(1)], sp[R4(2)], sn[R4(1)], sn[R4(2)],
+                                                   sn[R4(3)], eps);
+        {
+            const double2 a0 = sm.P[ty + 3][tx + 1], a1 = sm.P[ty + 3][tx + 2], a2 = sm.P[ty + 3][tx + 3],
+                          a3 = sm.P[ty + 3][tx + 4], a4 = sm.P[ty + 3][tx + 5], a5 = sm.P[ty + 3][tx + 6];
+            sm.FX[ty][tx + 1] = weno_face2<WENO, P, FD>(a0.x, a1.x, a2.x, a3.x, a4.x, a1.y, a2.y, a3.y,
+                                                    a4.y, a5.y, eps);
+        }
+        {
+            const double2 a0 = sm.P[ty + 1][tx + 3], a1 = sm.P[ty + 2][tx + 3], a2 = sm.P[ty + 3][tx + 3],
+                          a3 = sm.P[ty + 4][tx + 3], a4 = sm.P[ty + 5][tx + 3], a5 = sm.P[ty + 6][tx + 3];
+            sm.FY[ty + 1][tx] = weno_face2<WENO, P, FD>(a0.x, a1.x, a2.x, a3.x, a4.x, a1.y, a2.y, a3.y,
+                                                    a4.y, a5.y, eps);
+        }
+        // faces on the tile's low edges, by one wave that rotates with k
+        if (wave == k % NW && lane < TY + TX) {
+            double2 a0, a1, a2, a3, a4, a5;
+            if (lane < TY) {
+                const int r = lane + 3;
+                a0 = sm.P[r][0]; a1 = sm.P[r][1]; a2 = sm.P[r][2];
+                a3 = sm.P[r][3]; a4 = sm.P[r][4]; a5 = sm.P[r][5];
+            } else {
+                const int c = lane - TY + 3;
+                a0 = sm.P[0][c]; a1 = sm.P[1][c]; a2 = sm.P[2][c];
+                a3 = sm.P[3][c]; a4 = sm.P[4][c]; a5 = sm.P[5][c];
             }
-            {
-                const double2 a0 = sm.P[ty + 1][tx + 3], a1 = sm.P[ty + 2][tx + 3], a2 = sm.P[ty + 3][tx + 3],
-                              a3 = sm.P[ty + 4][tx + 3], a4 = sm.P[ty + 5][tx + 3], a5 = sm.P[ty + 6][tx + 3];
-                sm.FY[ty + 1][tx] = weno_face2<WENO, P, FD>(a0.x, a1.x, a2.x, a3.x, a4.x, a1.y, a2.y, a3.y,
-                                                        a4.y, a5.y, eps);
-            }
-            // faces on the tile's low edges, by one wave that rotates with k
-            if (wave == k % NW && lane < TY + TX) {
-                double2 a0, a1, a2, a3, a4, a5;
-                if (lane < TY) {
-                    const int r = lane + 3;
-                    a0 = sm.P[r][0]; a1 = sm.P[r][1]; a2 = sm.P[r][2];
-                    a3 = sm.P[r][3]; a4 = sm.P[r][4]; a5 = sm.P[r][5];
-                } else {
-                    const int c = lane - TY + 3;
-                    a0 = sm.P[0][c]; a1 = sm.P[1][c]; a2 = sm.P[2][c];
-                    a3 = sm.P[3][c]; a4 = sm.P[4][c]; a5 = sm.P[5][c];
-                }
-                const double f = weno_face2<WENO, P, FD>(a0.x, a1.x, a2.x, a3.x, a4.x, a1.y, a2.y, a3.y, a4.y,
-                                                     a5.y, eps);
-                if (lane < TY)
-                    sm.FX[lane][0] = f;
-                else
-                    sm.FY[0][lane - TY] = f;
-            }
-            __syncthreads();
-
-            double Lu = -div_by(sm.FX[ty][tx + 1] - sm.FX[ty][tx], g.dx, g.rdx);
-            Lu -= div_by(sm.FY[ty + 1][tx] - sm.FY[ty][tx], g.dy, g.rdy);
-            Lu -= div_by(fz - fz_prev, g.dz, g.rdz);
-            const double uo = (stage == 1) ? qu[R4(0)] : unv;
-            double unew;
-            if (stage == 1)
-                unew = uo + dt * Lu;
-            else if (stage == 2)
-                unew = 0.75 * uo + 0.25 * (qu[R4(0)] + dt * Lu);
+            const double f = weno_face2<WENO, P, FD>(a0.x, a1.x, a2.x, a3.x, a4.x, a1.y, a2.y, a3.y, a4.y,
+                                                 a5.y, eps);
+            if (lane < TY)
+                sm.FX[lane][0] = f;
             else
-                unew = div_by(p2fma(2.0, qu[R4(0)] + dt * Lu, uo), 3.0, 1.0 / 3.0);  // (uo + 2(..)) / 3
-            const T stored = static_cast<T>(unew);
-            out_ptr[plane * zi(k)] = stored;
-            if (ZG) {
-                if (k < 3) gh_lo[col + plane * static_cast<unsigned>(nz + 3 + k)] = stored;
-                if (k >= nz - 3) gh_hi[col + plane * static_cast<unsigned>(k - nz + 3)] = stored;
-            }
-            const unsigned long long b = abs_bits(static_cast<double>(stored));
-            vmax = b > vmax ? b : vmax;
+                sm.FY[0][lane - TY] = f;
+        }
+        __syncthreads();
 
-            fz_prev = fz;
+        double Lu = -div_by(sm.FX[ty][tx + 1] - sm.FX[ty][tx], g.dx, g.rdx);
+        Lu -= div_by(sm.FY[ty + 1][tx] - sm.FY[ty][tx], g.dy, g.rdy);
+        Lu -= div_by(fz - fz_prev, g.dz, g.rdz);
+        const double uo = (stage == 1) ? qu[R4(0)] : unv;
+        double unew;
+        if (stage == 1)
+            unew = uo + dt * Lu;
+        else if (stage == 2)
+            unew = 0.75 * uo + 0.25 * (qu[R4(0)] + dt * Lu);
+        else
+            unew = div_by(p2fma(2.0, qu[R4(0)] + dt * Lu, uo), 3.0, 1.0 / 3.0);  // (uo + 2(..)) / 3
+        const T stored = static_cast<T>(unew);
+        out_ptr[plane * zi(k)] = stored;
+        if (ZG) {
+            if (k < 3) gh_lo[col + plane * static_cast<unsigned>(nz + 3 + k)] = stored;
+            if (k >= nz - 3) gh_hi[col + plane * static_cast<unsigned>(k - nz + 3)] = stored;
+        }
+        const unsigned long long b = abs_bits(static_cast<double>(stored));
+        vmax = b > vmax ? b : vmax;
+
+        fz_prev = fz;
     };
     for (; kb < z1; kb += 8) {
         zstep(std::integral_constant<int, 0>{});

@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "mgad_common.hpp"
 
@@ -62,7 +63,7 @@ __device__ __forceinline__ bool wave_all(bool ok) { return __builtin_amdgcn_ball
 // AXIS 0: x lines (rows), 1: y lines, 2: z lines.  AXIS 0 writes Lu, 1 and 2 subtract.
 // px is the row pitch in elements (px >= nx; px = nx for unpitched arrays).
 // Along its line chunk a thread keeps the split fluxes of nodes i-3..i+3 and
-// the 13/12 terms of the sub-stencil centres in register queues, so each
+// the 13/12 terms of the sub-stencil centres in register rings, so each
 // node's terms are computed once, not once per face that reads them.
 // Lu -/= (F_{i+1/2} - F_{i-1/2})/d is div_by with rd = RN(1/d).  The next
 // node's u and Lu are loaded one face ahead.
@@ -109,47 +110,66 @@ __global__ __launch_bounds__(256) void k_adv_sweep(const double *__restrict__ u,
             lf_split(v, alpha, fp, fm);
     };
     auto load = [&](int i, double &fp, double &fm) { take(u[base + st * static_cast<unsigned>(i)], fp, fm); };
-    // queue: fp[q], fm[q] hold node i-3+q for the face being computed (node
-    // i0-3 is read only when the chunk's low face is computed); sp_c / sm_c
-    // are the 13/12 terms of f+ / f- centred on queue node c
-    double fp[7], fm[7];
+    // rings: fp/fm hold the split fluxes of nodes i-3..i+3 for the face being
+    // computed, node n in slot (n - i0 + 3) & 7 (node i0-3 is read only when
+    // the chunk's low face is computed); SP/SM the 13/12 terms of f+ / f-
+    // centred on node n in slot (n - i0 + 3) & 3.  The march is unrolled by
+    // 8, so every slot is a constant and no register moves shift the rings.
+    double fp[8], fm[8], SP[4], SM[4];
     fp[0] = fm[0] = 0.0;
     if (low_face) load(i0 - 3, fp[0], fm[0]);
 #pragma unroll
     for (int q = 1; q < 6; q++) load(i0 - 3 + q, fp[q], fm[q]);
-    double sp2 = sterm(fp[1], fp[2], fp[3]), sp3 = sterm(fp[2], fp[3], fp[4]);
-    double sm3 = sterm(fm[2], fm[3], fm[4]), sm4 = sterm(fm[3], fm[4], fm[5]);
+    SP[2] = sterm(fp[1], fp[2], fp[3]);
+    SP[3] = sterm(fp[2], fp[3], fp[4]);
+    SM[3] = sterm(fm[2], fm[3], fm[4]);
+    SM[0] = sterm(fm[3], fm[4], fm[5]);
     double fu_old = 0.0;
     if (low_face)
         fu_old = face_fd<WENO, P>(eps_ok && wave_all(ok), fp[0], fp[1], fp[2], fp[3], fp[4], fm[1], fm[2], fm[3],
-                                  fm[4], fm[5], sterm(fp[0], fp[1], fp[2]), sp2, sp3, sterm(fm[1], fm[2], fm[3]),
-                                  sm3, sm4, eps);
+                                  fm[4], fm[5], sterm(fp[0], fp[1], fp[2]), SP[2], SP[3], sterm(fm[1], fm[2], fm[3]),
+                                  SM[3], SM[0], eps);
     // u of node i+3 and Lu of node i are loaded one node ahead
     double u_next = u[base + st * static_cast<unsigned>(i0 + 3)];
     double lu_next = AXIS == 0 ? 0.0 : Lu[base + st * static_cast<unsigned>(i0)];
-    for (int i = i0; i < i1; i++) {
+    int ib = i0;
+    auto node = [&](auto S) {
+        constexpr int s = decltype(S)::value;
+        auto R8 = [](int q) { return (s + q) & 7; };
+        auto R4 = [](int q) { return (s + q) & 3; };
+        const int i = ib + s;
         const double v = u_next, lu = lu_next;
         if (i + 1 < i1) {
             u_next = u[base + st * static_cast<unsigned>(i + 4)];
             if (AXIS != 0) lu_next = Lu[base + st * static_cast<unsigned>(i + 1)];
         }
-        take(v, fp[6], fm[6]);
-        const double sp4 = sterm(fp[3], fp[4], fp[5]), sm5 = sterm(fm[4], fm[5], fm[6]);
-        const double fu = face_fd<WENO, P>(eps_ok && wave_all(ok), fp[1], fp[2], fp[3], fp[4], fp[5], fm[2], fm[3],
-                                           fm[4], fm[5], fm[6], sp2, sp3, sp4, sm3, sm4, sm5, eps);
+        take(v, fp[R8(6)], fm[R8(6)]);
+        SP[R4(4)] = sterm(fp[R8(3)], fp[R8(4)], fp[R8(5)]);
+        SM[R4(5)] = sterm(fm[R8(4)], fm[R8(5)], fm[R8(6)]);
+        const double fu = face_fd<WENO, P>(eps_ok && wave_all(ok), fp[R8(1)], fp[R8(2)], fp[R8(3)], fp[R8(4)],
+                                           fp[R8(5)], fm[R8(2)], fm[R8(3)], fm[R8(4)], fm[R8(5)], fm[R8(6)],
+                                           SP[R4(2)], SP[R4(3)], SP[R4(4)], SM[R4(3)], SM[R4(4)], SM[R4(5)], eps);
         const unsigned o = base + st * static_cast<unsigned>(i);
         const double q = div_by(fu - fu_old, d, rd);
         Lu[o] = AXIS == 0 ? -q : lu - q;
         fu_old = fu;
-        sp2 = sp3;
-        sp3 = sp4;
-        sm3 = sm4;
-        sm4 = sm5;
-#pragma unroll
-        for (int q2 = 0; q2 < 6; q2++) {
-            fp[q2] = fp[q2 + 1];
-            fm[q2] = fm[q2 + 1];
-        }
+    };
+    for (; ib < i1; ib += 8) {
+        node(std::integral_constant<int, 0>{});
+        if (ib + 1 >= i1) break;
+        node(std::integral_constant<int, 1>{});
+        if (ib + 2 >= i1) break;
+        node(std::integral_constant<int, 2>{});
+        if (ib + 3 >= i1) break;
+        node(std::integral_constant<int, 3>{});
+        if (ib + 4 >= i1) break;
+        node(std::integral_constant<int, 4>{});
+        if (ib + 5 >= i1) break;
+        node(std::integral_constant<int, 5>{});
+        if (ib + 6 >= i1) break;
+        node(std::integral_constant<int, 6>{});
+        if (ib + 7 >= i1) break;
+        node(std::integral_constant<int, 7>{});
     }
 }
 
