@@ -312,18 +312,24 @@ typedef struct {
     size_t workspace_bytes;
     int nprob, nitems;      /* problems, (problem, tile) work items */
     int grid;               /* persistent blocks per stage launch */
-    int kchunk;             /* planes per work item (autotuned if 0 given) */
+    int kchunk;             /* planes per work item, or MGAD_BATCH_SPLIT */
     int precision;          /* MGAD_F64 or MGAD_F32 */
     mgad_scheme scheme;
 } mgad_batch;
 
+/* kchunk of the split work list: one item per 16x16 column tile (all its
+ * planes), the items' planes cut into one equal share per resident block. */
+#define MGAD_BATCH_SPLIT (-1)
+
 /* Device bytes the workspace of this batch needs (upper bound over the
- * chunk depths the autotuner tries). */
+ * work lists the autotuner tries). */
 size_t mgad_batch_workspace_bytes(int nprob, const mgad_problem *probs);
 
 /* Builds the work list in the caller's workspace and initialises every
  * problem's ctrl at t0 (u of each problem must hold its initial field).
- * kchunk 0 autotunes the planes per work item by timing the batched stage.
+ * kchunk > 0 fixes the planes per work item, MGAD_BATCH_SPLIT picks the
+ * split list, and 0 autotunes: it times the batched stage with the split list
+ * and with 8, 16 and 32 planes per item and keeps the fastest.
  * Synchronises the stream. */
 int mgad_batch_init(hipStream_t stream, int precision,
                     const mgad_scheme *scheme, int nprob,

@@ -16,6 +16,7 @@ WENO_Z, WENO_JS = 0, 1
 LF_LOCAL, LF_GLOBAL = 0, 1
 CTRL_BYTES = 256
 F64, F32 = 0, 1
+BATCH_SPLIT = -1  # kchunk of the split batch work list
 
 # Every symbol include/mgad.h declares.
 EXPORTS = (

@@ -107,12 +107,13 @@ struct TileSmem {
 // ZG, planes 0..2 are also stored into the top ghost planes of the slab set
 // at gh_lo and planes nz-3..nz-1 into the bottom ghost planes at gh_hi (the
 // neighbours' copies of them, so no separate exchange is needed).
+// stage_cols: the tile whose first column is (x0, y0), planes z0 .. z1-1
+// (0 <= z0 < z1 <= nz).
 template <class T, int WENO, int P, int TX, int TY, bool FD, bool ZG = false>
-__device__ __forceinline__ void stage_tile(const T *__restrict__ uin, const T *un, T *uout,
-                                           const GridArgs &g, int stage, unsigned tile, double dt,
-                                           double alpha, double eps, TileSmem<TX, TY> &sm,
-                                           unsigned long long *amax_slot, T *gh_lo = nullptr,
-                                           T *gh_hi = nullptr) {
+__device__ __forceinline__ void stage_cols(const T *__restrict__ uin, const T *un, T *uout,
+                                           const GridArgs &g, int stage, int x0, int y0, int z0, int z1,
+                                           double dt, double alpha, double eps, TileSmem<TX, TY> &sm,
+                                           unsigned long long *amax_slot, T *gh_lo, T *gh_hi) {
     constexpr int NT = TX * TY;
     constexpr int NW = NT / 64;
     static_assert(NT % 64 == 0, "whole waves");
@@ -120,13 +121,6 @@ __device__ __forceinline__ void stage_tile(const T *__restrict__ uin, const T *u
     static_assert(TX + TY <= 64, "edge faces fit in one wave");
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     const int tid = threadIdx.x;
-    const int ntx = nx / TX, nty = ny / TY;
-    const int bx = tile % ntx;
-    const int by = (tile / ntx) % nty;
-    const int bz = tile / (ntx * nty);
-    const int x0 = bx * TX, y0 = by * TY;
-    const int z0 = bz * g.kchunk;
-    const int z1 = min(z0 + g.kchunk, nz);
     const int tx = tid % TX, ty = tid / TX;
     const int wave = tid >> 6, lane = tid & 63;
     const unsigned plane = static_cast<unsigned>(nx) * static_cast<unsigned>(ny);
@@ -303,6 +297,23 @@ __device__ __forceinline__ void stage_tile(const T *__restrict__ uin, const T *u
         for (int w = 1; w < NW; w++) m = sm.red[w] > m ? sm.red[w] : m;
         atomicMax(amax_slot, m);
     }
+}
+
+// stage_cols for block `tile` of the chunked decomposition: tiles x-fastest,
+// then y, then z chunks of g.kchunk planes.
+template <class T, int WENO, int P, int TX, int TY, bool FD, bool ZG = false>
+__device__ __forceinline__ void stage_tile(const T *__restrict__ uin, const T *un, T *uout,
+                                           const GridArgs &g, int stage, unsigned tile, double dt,
+                                           double alpha, double eps, TileSmem<TX, TY> &sm,
+                                           unsigned long long *amax_slot, T *gh_lo = nullptr,
+                                           T *gh_hi = nullptr) {
+    const int ntx = g.nx / TX, nty = g.ny / TY;
+    const int bx = tile % ntx;
+    const int by = (tile / ntx) % nty;
+    const int bz = tile / (ntx * nty);
+    const int z0 = bz * g.kchunk;
+    stage_cols<T, WENO, P, TX, TY, FD, ZG>(uin, un, uout, g, stage, bx * TX, by * TY, z0, min(z0 + g.kchunk, g.nz),
+                                           dt, alpha, eps, sm, amax_slot, gh_lo, gh_hi);
 }
 
 int default_kchunk(int nx, int ny, int nz, int tx, int ty);

@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 import oracle as O
-from multigpu_advectiondiffusion_amd import solvers
+from multigpu_advectiondiffusion_amd import _capi, solvers
 
 pytestmark = pytest.mark.gpu
 
@@ -22,13 +22,18 @@ def alone(shape, u0, scheme, nsteps, tend=10.0):
     return s.solution(), s.state()
 
 
+# kchunk -1 is the split list: SIZES hold 1712 tile planes, so the ~1024
+# shares of a full device are one or two planes long and most pieces start or
+# end inside a column.
 @pytest.mark.parametrize("scheme", ["Z2", "JS"])
-@pytest.mark.parametrize("kchunk", [0, 8, 16])
+@pytest.mark.parametrize("kchunk", [0, 8, 16, _capi.BATCH_SPLIT])
 def test_batch_matches_single_problems(hipdev, scheme, kchunk):
     b = solvers.BurgersBatch(SIZES, scheme=scheme)
     u0s = [solvers.ic_fourier(s, seed=40 + i) for i, s in enumerate(SIZES)]
     b.set_initial(u0s, tend=10.0, kchunk=kchunk)
-    assert b.batch.kchunk in (8, 16, 32) and b.batch.nitems > 0
+    assert b.batch.kchunk in (_capi.BATCH_SPLIT, 8, 16, 32) and b.batch.nitems > 0
+    if kchunk:
+        assert b.batch.kchunk == kchunk
     b.steps(4, graph=True)
     got = b.solutions()
     sts = b.states()
@@ -51,11 +56,12 @@ def test_batch_serial_path_and_oracle(hipdev):
         assert np.array_equal(got[i], ref), i
 
 
-def test_batch_stops_each_problem_at_tend(hipdev):
+@pytest.mark.parametrize("kchunk", [16, _capi.BATCH_SPLIT])
+def test_batch_stops_each_problem_at_tend(hipdev, kchunk):
     sizes = [16, 32, 48]
     b = solvers.BurgersBatch(sizes, scheme="JS")
     u0s = [solvers.ic_fourier(s, seed=70 + i) for i, s in enumerate(sizes)]
-    b.set_initial(u0s, tend=0.2)
+    b.set_initial(u0s, tend=0.2, kchunk=kchunk)
     b.steps(200, graph=True)
     sts = b.states()
     got = b.solutions()
