@@ -317,9 +317,12 @@ typedef struct {
     mgad_scheme scheme;
 } mgad_batch;
 
-/* kchunk of the split work list: one item per 16x16 column tile (all its
- * planes), the items' planes cut into one equal share per resident block. */
-#define MGAD_BATCH_SPLIT (-1)
+/* kchunk of the split work lists: one item per column tile (all its planes),
+ * the items' planes cut into one equal share per resident block.  Tiles are
+ * 16x16 columns, or 32x16 for the wide list (where nx % 32 == 16, the last
+ * tile of a row reaches past nx and stores only its first 16 columns). */
+#define MGAD_BATCH_SPLIT      (-1)
+#define MGAD_BATCH_SPLIT_WIDE (-2)
 
 /* Device bytes the workspace of this batch needs (upper bound over the
  * work lists the autotuner tries). */
@@ -327,9 +330,9 @@ size_t mgad_batch_workspace_bytes(int nprob, const mgad_problem *probs);
 
 /* Builds the work list in the caller's workspace and initialises every
  * problem's ctrl at t0 (u of each problem must hold its initial field).
- * kchunk > 0 fixes the planes per work item, MGAD_BATCH_SPLIT picks the
- * split list, and 0 autotunes: it times the batched stage with the split list
- * and with 8, 16 and 32 planes per item and keeps the fastest.
+ * kchunk > 0 fixes the planes per work item, MGAD_BATCH_SPLIT(_WIDE) picks
+ * a split list, and 0 autotunes: it times the batched stages with both split
+ * lists and with 8, 16 and 32 planes per item and keeps the fastest.
  * Synchronises the stream. */
 int mgad_batch_init(hipStream_t stream, int precision,
                     const mgad_scheme *scheme, int nprob,

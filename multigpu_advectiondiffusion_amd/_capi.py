@@ -16,7 +16,7 @@ WENO_Z, WENO_JS = 0, 1
 LF_LOCAL, LF_GLOBAL = 0, 1
 CTRL_BYTES = 256
 F64, F32 = 0, 1
-BATCH_SPLIT = -1  # kchunk of the split batch work list
+BATCH_SPLIT, BATCH_SPLIT_WIDE = -1, -2  # kchunk of the split batch work lists (16x16 / 32x16 tiles)
 
 # Every symbol include/mgad.h declares.
 EXPORTS = (

@@ -40,22 +40,25 @@ struct BatchProblem {
 // One piece of a work item: planes z0 .. z1-1 of the tile's columns.  The
 // piece holding the problem's first column and plane 0 leads its stage
 // control.
-template <class T, int WENO, int P>
+// Tiles are TX x 16 columns; a 32-wide tile row of a problem with nx % 32 == 16
+// ends in a tile that reaches past nx (its last 16 columns are not stored).
+template <class T, int WENO, int P, int TX>
 __device__ __forceinline__ void batch_piece(const BatchProblem &bp, unsigned tile, int z0, int z1, double eps,
-                                            int stage, TileSmem<kBT, kBT> &sm) {
+                                            int stage, TileSmem<TX, kBT> &sm) {
+    constexpr bool XW = TX != kBT;
     double dt, alpha;
     if (!stage_control(bp.ctrl, stage, bp.g.dx, tile == 0 && z0 == 0 && threadIdx.x == 0, dt, alpha)) return;
     const T *u = static_cast<const T *>(bp.u);
     const T *uin = stage == 1 ? u : (stage == 2 ? static_cast<const T *>(bp.w1) : static_cast<const T *>(bp.w2));
     T *uout = stage == 1 ? static_cast<T *>(bp.w1) : (stage == 2 ? static_cast<T *>(bp.w2) : static_cast<T *>(bp.u));
-    const int ntx = bp.g.nx / kBT;
-    const int x0 = static_cast<int>(tile % ntx) * kBT, y0 = static_cast<int>(tile / ntx) * kBT;
+    const int ntx = (bp.g.nx + TX - 1) / TX;
+    const int x0 = static_cast<int>(tile % ntx) * TX, y0 = static_cast<int>(tile / ntx) * kBT;
     if (fast_division_ok<WENO>(alpha, eps))
-        stage_cols<T, WENO, P, kBT, kBT, true>(uin, u, uout, bp.g, stage, x0, y0, z0, z1, dt, alpha, eps, sm,
-                                               &bp.ctrl->amax[stage % 3], nullptr, nullptr);
+        stage_cols<T, WENO, P, TX, kBT, true, false, XW>(uin, u, uout, bp.g, stage, x0, y0, z0, z1, dt, alpha, eps,
+                                                         sm, &bp.ctrl->amax[stage % 3], nullptr, nullptr);
     else
-        stage_cols<T, WENO, P, kBT, kBT, false>(uin, u, uout, bp.g, stage, x0, y0, z0, z1, dt, alpha, eps, sm,
-                                                &bp.ctrl->amax[stage % 3], nullptr, nullptr);
+        stage_cols<T, WENO, P, TX, kBT, false, false, XW>(uin, u, uout, bp.g, stage, x0, y0, z0, z1, dt, alpha, eps,
+                                                          sm, &bp.ctrl->amax[stage % 3], nullptr, nullptr);
 }
 
 // Chunked list: item i is tile item_tile[i] of problem item_prob[i], tiles
@@ -73,7 +76,7 @@ __global__ __launch_bounds__(kBT *kBT, 4) void k_burgers_batch(const BatchProble
         const unsigned ntile = static_cast<unsigned>(bp.g.nx / kBT) * static_cast<unsigned>(bp.g.ny / kBT);
         const unsigned tile = item_tile[i] % ntile;
         const int z0 = static_cast<int>(item_tile[i] / ntile) * bp.g.kchunk;
-        batch_piece<T, WENO, P>(bp, tile, z0, min(z0 + bp.g.kchunk, bp.g.nz), eps, stage, sm);
+        batch_piece<T, WENO, P, kBT>(bp, tile, z0, min(z0 + bp.g.kchunk, bp.g.nz), eps, stage, sm);
     }
 }
 
@@ -81,21 +84,22 @@ __global__ __launch_bounds__(kBT *kBT, 4) void k_burgers_batch(const BatchProble
 // planes, taken in list order, are cut into G shares of equal plane count;
 // block b (remapped as above) runs share s from shares[s] to shares[s + 1]
 // (item, plane) pairs, shares[G] = (nitems, 0).  Every block does the same
-// number of tile planes, whatever the mix of problem sizes.
-template <class T, int WENO, int P>
-__global__ __launch_bounds__(kBT *kBT, 4) void k_burgers_batch_split(const BatchProblem *__restrict__ probs,
-                                                                     const unsigned *__restrict__ item_prob,
-                                                                     const unsigned *__restrict__ item_tile,
-                                                                     const uint2 *__restrict__ shares, int nitems,
-                                                                     double eps, int stage) {
-    __shared__ TileSmem<kBT, kBT> sm;
+// number of tile planes, whatever the mix of problem sizes.  Tiles are TX x 16
+// (TX = 16 or 32) columns.
+template <class T, int WENO, int P, int TX>
+__global__ __launch_bounds__(TX *kBT, 4) void k_burgers_batch_split(const BatchProblem *__restrict__ probs,
+                                                                    const unsigned *__restrict__ item_prob,
+                                                                    const unsigned *__restrict__ item_tile,
+                                                                    const uint2 *__restrict__ shares, int nitems,
+                                                                    double eps, int stage) {
+    __shared__ TileSmem<TX, kBT> sm;
     const unsigned s = xcd_tile(blockIdx.x, gridDim.x);
     const uint2 a = shares[s], e = shares[s + 1];
     for (unsigned i = a.x; i <= e.x && i < static_cast<unsigned>(nitems); i++) {
         const BatchProblem &bp = probs[item_prob[i]];
         const int z0 = i == a.x ? static_cast<int>(a.y) : 0;
         const int z1 = i == e.x ? static_cast<int>(e.y) : bp.g.nz;
-        if (z0 < z1) batch_piece<T, WENO, P>(bp, item_tile[i], z0, z1, eps, stage, sm);
+        if (z0 < z1) batch_piece<T, WENO, P, TX>(bp, item_tile[i], z0, z1, eps, stage, sm);
     }
 }
 
@@ -115,12 +119,20 @@ BatchKernel<T> pick_batch(const mgad_scheme &sc) {
     return nullptr;
 }
 
-template <class T>
+template <class T, int TX>
 SplitKernel<T> pick_split(const mgad_scheme &sc) {
-    if (sc.weno == MGAD_WENO_Z && sc.p == 1) return k_burgers_batch_split<T, kWenoZ, 1>;
-    if (sc.weno == MGAD_WENO_Z && sc.p == 2) return k_burgers_batch_split<T, kWenoZ, 2>;
-    if (sc.weno == MGAD_WENO_JS) return k_burgers_batch_split<T, kWenoJS, 2>;
+    if (sc.weno == MGAD_WENO_Z && sc.p == 1) return k_burgers_batch_split<T, kWenoZ, 1, TX>;
+    if (sc.weno == MGAD_WENO_Z && sc.p == 2) return k_burgers_batch_split<T, kWenoZ, 2, TX>;
+    if (sc.weno == MGAD_WENO_JS) return k_burgers_batch_split<T, kWenoJS, 2, TX>;
     return nullptr;
+}
+
+bool is_split(int kchunk) { return kchunk == MGAD_BATCH_SPLIT || kchunk == MGAD_BATCH_SPLIT_WIDE; }
+// tile width of a work list: 32 for the wide split list, else 16
+int tile_w(int kchunk) { return kchunk == MGAD_BATCH_SPLIT_WIDE ? 2 * kBT : kBT; }
+// column tiles of one problem in a list of tile width tw
+long col_tiles(const mgad_problem &p, int tw) {
+    return static_cast<long>((p.nx + tw - 1) / tw) * (p.ny / kBT);
 }
 
 // descriptors | item_prob[nitems] | item_tile[nitems] | shares[nshares + 1] (split list only)
@@ -129,22 +141,23 @@ size_t ws_bytes(int nprob, long nitems, long nshares) {
            (nshares > 0 ? sizeof(uint2) * static_cast<size_t>(nshares + 1) : 0);
 }
 
-// Items of the chunked list with kchunk planes per item, or of the split list
-// (one per column tile) for kchunk == MGAD_BATCH_SPLIT.
+// Items of the chunked list with kchunk planes per item, or of a split list
+// (one per column tile).
 long count_items(int nprob, const mgad_problem *p, int kchunk) {
     long n = 0;
     for (int i = 0; i < nprob; i++)
-        n += static_cast<long>(p[i].nx / kBT) * (p[i].ny / kBT) *
-             (kchunk == MGAD_BATCH_SPLIT ? 1 : (p[i].nz + kchunk - 1) / kchunk);
+        n += col_tiles(p[i], tile_w(kchunk)) * (is_split(kchunk) ? 1 : (p[i].nz + kchunk - 1) / kchunk);
     return n;
 }
 
-int resident_blocks() {
+// Blocks of tw x 16 threads the device keeps resident at the kernels' 4
+// waves per SIMD: 16 waves per CU.
+int resident_blocks(int tw) {
     int dev = 0, cus = 0;
-    if (hipGetDevice(&dev) != hipSuccess) return 1024;
-    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
-        return 1024;
-    return cus * 4;  // four 4-wave blocks per CU at the kernel's 4 waves/SIMD occupancy
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+        cus = 256;
+    return cus * 16 / (tw * kBT / 64);
 }
 
 bool problems_ok(int nprob, const mgad_problem *p) {
@@ -181,10 +194,11 @@ void cut_shares(const std::vector<unsigned> &planes, long nshares, uint2 *shares
 int write_workspace(hipStream_t stream, int nprob, const mgad_problem *p, int kchunk, void *ws, size_t ws_size,
                     mgad_batch &b) {
     const long nitems = count_items(nprob, p, kchunk);
+    const bool split = is_split(kchunk);
+    const int tw = tile_w(kchunk);
     long planes_total = 0;
-    for (int i = 0; i < nprob; i++) planes_total += static_cast<long>(p[i].nx / kBT) * (p[i].ny / kBT) * p[i].nz;
-    const bool split = kchunk == MGAD_BATCH_SPLIT;
-    const long grid = std::max(1l, std::min(split ? planes_total : nitems, static_cast<long>(resident_blocks())));
+    for (int i = 0; i < nprob; i++) planes_total += col_tiles(p[i], tw) * p[i].nz;
+    const long grid = std::max(1l, std::min(split ? planes_total : nitems, static_cast<long>(resident_blocks(tw))));
     const long nshares = split ? grid : 0;
     if (ws_bytes(nprob, nitems, nshares) > ws_size || nitems >= (1l << 31)) {
         set_error(MGAD_ERR_INVALID_ARG);
@@ -203,8 +217,7 @@ int write_workspace(hipStream_t stream, int nprob, const mgad_problem *p, int kc
         bp[i].ctrl = static_cast<Ctrl *>(p[i].ctrl);
         bp[i].g = make_grid(p[i].nx, p[i].ny, p[i].nz, p[i].dx, p[i].dy, p[i].dz,
                             split ? static_cast<int>(p[i].nz) : kchunk);
-        const long nt = static_cast<long>(p[i].nx / kBT) * (p[i].ny / kBT) *
-                        (split ? 1 : (p[i].nz + kchunk - 1) / kchunk);
+        const long nt = col_tiles(p[i], tw) * (split ? 1 : (p[i].nz + kchunk - 1) / kchunk);
         for (long t = 0; t < nt; t++, k++) {
             ip[k] = static_cast<unsigned>(i);
             it[k] = static_cast<unsigned>(t);
@@ -225,13 +238,14 @@ int launch_batch_stage(hipStream_t stream, const mgad_batch &b, int stage) {
     const BatchProblem *probs = static_cast<const BatchProblem *>(b.workspace);
     const unsigned *ip = reinterpret_cast<const unsigned *>(probs + b.nprob);
     const unsigned *it = ip + b.nitems;
-    if (b.kchunk == MGAD_BATCH_SPLIT) {
-        SplitKernel<T> fn = pick_split<T>(b.scheme);
+    if (is_split(b.kchunk)) {
+        const int tw = tile_w(b.kchunk);
+        SplitKernel<T> fn = tw == kBT ? pick_split<T, kBT>(b.scheme) : pick_split<T, 2 * kBT>(b.scheme);
         if (!fn) {
             set_error(MGAD_ERR_UNSUPPORTED);
             return MGAD_ERR_UNSUPPORTED;
         }
-        hipLaunchKernelGGL(fn, dim3(b.grid), dim3(kBT * kBT), 0, stream, probs, ip, it,
+        hipLaunchKernelGGL(fn, dim3(b.grid), dim3(tw * kBT), 0, stream, probs, ip, it,
                            reinterpret_cast<const uint2 *>(it + b.nitems), b.nitems, b.scheme.eps, stage);
     } else {
         BatchKernel<T> fn = pick_batch<T>(b.scheme);
@@ -336,14 +350,14 @@ extern "C" size_t mgad_batch_workspace_bytes(int nprob, const mgad_problem *prob
     if (nprob <= 0 || !probs) return 0;
     // the chunked list of the smallest chunk the tuner tries holds at least
     // as many items as the split list, which adds one share per block
-    return ws_bytes(nprob, count_items(nprob, probs, 4), resident_blocks());
+    return ws_bytes(nprob, count_items(nprob, probs, 4), resident_blocks(kBT));
 }
 
 extern "C" int mgad_batch_init(hipStream_t stream, int precision, const mgad_scheme *scheme, int nprob,
                                const mgad_problem *probs, void *workspace, size_t workspace_bytes, int kchunk,
                                double t0, double tend, double cfl, mgad_batch *out) {
     if (!scheme || !out || !workspace || (precision != MGAD_F64 && precision != MGAD_F32) ||
-        !problems_ok(nprob, probs) || (kchunk < 0 && kchunk != MGAD_BATCH_SPLIT)) {
+        !problems_ok(nprob, probs) || (kchunk < 0 && !is_split(kchunk))) {
         set_error(MGAD_ERR_INVALID_ARG);
         return MGAD_ERR_INVALID_ARG;
     }
@@ -362,7 +376,7 @@ extern "C" int mgad_batch_init(hipStream_t stream, int precision, const mgad_sch
     if (kchunk != 0)
         cands.push_back(kchunk);
     else
-        cands = {MGAD_BATCH_SPLIT, 8, 16, 32};
+        cands = {MGAD_BATCH_SPLIT_WIDE, MGAD_BATCH_SPLIT, 8, 16, 32};
     int best_kc = cands[0];
     if (cands.size() > 1) {
         // autotune: time stages 1 and 2 of a step (u -> w1 -> w2; they leave
@@ -375,7 +389,7 @@ extern "C" int mgad_batch_init(hipStream_t stream, int precision, const mgad_sch
         for (int pass = 0; pass < 2 && !st; pass++) {
             for (size_t c = 0; c < cands.size(); c++) {
                 const int kc = cands[c];
-                const long nshares = kc == MGAD_BATCH_SPLIT ? resident_blocks() : 0;
+                const long nshares = is_split(kc) ? resident_blocks(tile_w(kc)) : 0;
                 if (ws_bytes(nprob, count_items(nprob, probs, kc), nshares) > workspace_bytes) continue;
                 if ((st = write_workspace(stream, nprob, probs, kc, workspace, workspace_bytes, b))) break;
                 st = init_ctrls(stream, precision, nprob, probs, t0, 1e300, cfl);

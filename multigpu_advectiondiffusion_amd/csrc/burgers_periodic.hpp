@@ -99,17 +99,20 @@ struct TileSmem {
     unsigned long long red[TX * TY / 64];
 };
 
-// One tile x z-chunk of one stage; ends with one atomicMax per block into the
-// stage's max|u| slot.  FD: fast_division_ok<WENO>(alpha, eps) holds.
+// One stage over the tile whose first column is (x0, y0), planes z0 .. z1-1
+// (0 <= z0 < z1 <= nz); ends with one atomicMax per block into the stage's
+// max|u| slot.  FD: fast_division_ok<WENO>(alpha, eps) holds.
 // ZG: the field is one z slab of nz planes stored between 3 ghost planes
 // below and 3 above (plane k at storage plane k + 3, read for k in -3..nz+2,
 // written for k in 0..nz-1); otherwise z is periodic over nz planes.  With
 // ZG, planes 0..2 are also stored into the top ghost planes of the slab set
 // at gh_lo and planes nz-3..nz-1 into the bottom ghost planes at gh_hi (the
 // neighbours' copies of them, so no separate exchange is needed).
-// stage_cols: the tile whose first column is (x0, y0), planes z0 .. z1-1
-// (0 <= z0 < z1 <= nz).
-template <class T, int WENO, int P, int TX, int TY, bool FD, bool ZG = false>
+// XW: the tile may reach past nx (x0 + TX > nx); its columns x >= nx stand
+// for x mod nx and are computed like the others, so every face of the real
+// columns is there, but are neither stored nor counted in max|u| (in stage 3
+// their u^n may already hold the owner's new value).
+template <class T, int WENO, int P, int TX, int TY, bool FD, bool ZG = false, bool XW = false>
 __device__ __forceinline__ void stage_cols(const T *__restrict__ uin, const T *un, T *uout,
                                            const GridArgs &g, int stage, int x0, int y0, int z0, int z1,
                                            double dt, double alpha, double eps, TileSmem<TX, TY> &sm,
@@ -119,12 +122,18 @@ __device__ __forceinline__ void stage_cols(const T *__restrict__ uin, const T *u
     static_assert(NT % 64 == 0, "whole waves");
     static_assert(6 * TX + 6 * TY <= NT, "one halo node per thread at most");
     static_assert(TX + TY <= 64, "edge faces fit in one wave");
+    static_assert(!(XW && ZG), "x-wrapped tiles are for periodic fields");
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     const int tid = threadIdx.x;
     const int tx = tid % TX, ty = tid / TX;
     const int wave = tid >> 6, lane = tid & 63;
     const unsigned plane = static_cast<unsigned>(nx) * static_cast<unsigned>(ny);
-    const unsigned col = static_cast<unsigned>(x0 + tx) + static_cast<unsigned>(nx) * (y0 + ty);
+    // x index of a column or halo node at x0 + i (i >= -3): wrapped once by
+    // periodicity, or reduced mod nx when the tile reaches past nx
+    auto xmod = [&](int i) { return XW ? (i + nx) % nx : wrapi(i, nx); };
+    const bool own_col = !XW || x0 + tx < nx;
+    const unsigned col =
+        static_cast<unsigned>(XW ? xmod(x0 + tx) : x0 + tx) + static_cast<unsigned>(nx) * (y0 + ty);
     // storage plane of plane k: zw for k in -3..nz+3 (wrapped when periodic), zi for 0 <= k < nz
     auto zw = [&](int k) { return static_cast<unsigned>(ZG ? k + 3 : wrapi(k, nz)); };
     auto zi = [&](int k) { return static_cast<unsigned>(ZG ? k + 3 : k); };
@@ -137,13 +146,14 @@ __device__ __forceinline__ void stage_cols(const T *__restrict__ uin, const T *u
         const int r = tid / 6, c = tid % 6;
         hl_r = r + 3;
         hl_c = c < 3 ? c : TX + c;
-        hcol = static_cast<unsigned>(wrapi(x0 + hl_c - 3, nx)) + static_cast<unsigned>(nx) * (y0 + r);
+        hcol = static_cast<unsigned>(xmod(x0 + hl_c - 3)) + static_cast<unsigned>(nx) * (y0 + r);
     } else if (has_halo) {
         const int h = tid - 6 * TY;
         const int c = h % TX, q = h / TX;
         hl_r = q < 3 ? q : TY + q;
         hl_c = c + 3;
-        hcol = static_cast<unsigned>(x0 + c) + static_cast<unsigned>(nx) * wrapi(y0 + hl_r - 3, ny);
+        hcol = static_cast<unsigned>(XW ? xmod(x0 + c) : x0 + c) +
+               static_cast<unsigned>(nx) * wrapi(y0 + hl_r - 3, ny);
     }
 
     // z register rings (see the march below): qp/qm hold the split fluxes of
@@ -259,12 +269,12 @@ __device__ __forceinline__ void stage_cols(const T *__restrict__ uin, const T *u
         else
             unew = div_by(p2fma(2.0, qu[R4(0)] + dt * Lu, uo), 3.0, 1.0 / 3.0);  // (uo + 2(..)) / 3
         const T stored = static_cast<T>(unew);
-        out_ptr[plane * zi(k)] = stored;
+        if (own_col) out_ptr[plane * zi(k)] = stored;
         if (ZG) {
             if (k < 3) gh_lo[col + plane * static_cast<unsigned>(nz + 3 + k)] = stored;
             if (k >= nz - 3) gh_hi[col + plane * static_cast<unsigned>(k - nz + 3)] = stored;
         }
-        const unsigned long long b = abs_bits(static_cast<double>(stored));
+        const unsigned long long b = own_col ? abs_bits(static_cast<double>(stored)) : 0ull;
         vmax = b > vmax ? b : vmax;
 
         fz_prev = fz;

@@ -22,16 +22,17 @@ def alone(shape, u0, scheme, nsteps, tend=10.0):
     return s.solution(), s.state()
 
 
-# kchunk -1 is the split list: SIZES hold 1712 tile planes, so the ~1024
-# shares of a full device are one or two planes long and most pieces start or
-# end inside a column.
+# kchunk -1 / -2 are the split lists: SIZES hold 1712 16x16 tile planes, so
+# the ~1024 shares of a full device are one or two planes long and most pieces
+# start or end inside a column.  With 32x16 tiles, the 16- and 48-wide problems
+# end each tile row in a tile that reaches past nx.
 @pytest.mark.parametrize("scheme", ["Z2", "JS"])
-@pytest.mark.parametrize("kchunk", [0, 8, 16, _capi.BATCH_SPLIT])
+@pytest.mark.parametrize("kchunk", [0, 8, 16, _capi.BATCH_SPLIT, _capi.BATCH_SPLIT_WIDE])
 def test_batch_matches_single_problems(hipdev, scheme, kchunk):
     b = solvers.BurgersBatch(SIZES, scheme=scheme)
     u0s = [solvers.ic_fourier(s, seed=40 + i) for i, s in enumerate(SIZES)]
     b.set_initial(u0s, tend=10.0, kchunk=kchunk)
-    assert b.batch.kchunk in (_capi.BATCH_SPLIT, 8, 16, 32) and b.batch.nitems > 0
+    assert b.batch.kchunk in (_capi.BATCH_SPLIT_WIDE, _capi.BATCH_SPLIT, 8, 16, 32) and b.batch.nitems > 0
     if kchunk:
         assert b.batch.kchunk == kchunk
     b.steps(4, graph=True)
@@ -56,7 +57,7 @@ def test_batch_serial_path_and_oracle(hipdev):
         assert np.array_equal(got[i], ref), i
 
 
-@pytest.mark.parametrize("kchunk", [16, _capi.BATCH_SPLIT])
+@pytest.mark.parametrize("kchunk", [16, _capi.BATCH_SPLIT, _capi.BATCH_SPLIT_WIDE])
 def test_batch_stops_each_problem_at_tend(hipdev, kchunk):
     sizes = [16, 32, 48]
     b = solvers.BurgersBatch(sizes, scheme="JS")
