@@ -1,6 +1,7 @@
 """Measurement studies of the non-headline configurations.
 
     python -m multigpu_advectiondiffusion_amd.studies batched   [--steps 200]
+    python -m multigpu_advectiondiffusion_amd.studies batched_trace [--steps 20]
     python -m multigpu_advectiondiffusion_amd.studies fp32      [--n 512]
     python -m multigpu_advectiondiffusion_amd.studies adjoint   [--n 128]
     python -m multigpu_advectiondiffusion_amd.studies reference
@@ -448,15 +449,32 @@ def adjoint_trace(n: int = 128, steps: int = 100) -> dict:
     return {"study": "adjoint_trace", "grid": n, "steps": nst, "J": J}
 
 
+def batched_trace(steps: int = 20, nprob: int = 64, scheme: str = "Z2") -> dict:
+    """The batched path alone on the batched study's 64 problems: the
+    autotuner's timing runs, then `steps` hipGraph-replayed steps with the
+    work list it picked (the workload of the batched kernel trace)."""
+    sizes = batch_sizes(nprob)
+    u0s = [solvers.ic_fourier(n, seed=4 + i) for i, n in enumerate(sizes)]
+    b = solvers.BurgersBatch(sizes, scheme=scheme)
+    b.set_initial(u0s, tend=float("inf"), kchunk=0)
+    b.steps(steps, graph=True)
+    b.stream.synchronize()
+    return {"study": "batched_trace", "problems": nprob, "points": b.npts, "steps": steps,
+            "kchunk_batched": b.batch.kchunk, "grid": b.batch.grid, "work_items": b.batch.nitems}
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("study",
-                    choices=["batched", "fp32", "adjoint", "adjoint_trace", "reference", "heat", "slab", "weno7"])
+                    choices=["batched", "batched_trace", "fp32", "adjoint", "adjoint_trace", "reference", "heat", "slab",
+                             "weno7"])
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--n", type=int, default=512)
     args = ap.parse_args(argv)
     if args.study == "batched":
         r = batched(args.steps)
+    elif args.study == "batched_trace":
+        r = batched_trace(args.steps if args.steps != 200 else 20)
     elif args.study == "reference":
         r = reference_cases()
     elif args.study == "heat":
