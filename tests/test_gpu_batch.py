@@ -44,6 +44,23 @@ def test_batch_matches_single_problems(hipdev, scheme, kchunk):
         assert sts[i].steps == 4 and sts[i].t == st.t
 
 
+@pytest.mark.parametrize("kchunk", [16, _capi.BATCH_SPLIT, _capi.BATCH_SPLIT_WIDE])
+def test_batch_fp32_matches_single_problems(hipdev, kchunk):
+    """FP32 storage: the batched stages equal each FP32 problem advanced alone."""
+    sizes = [32, 48, (16, 32, 24)]
+    b = solvers.BurgersBatch(sizes, scheme="Z2", precision="f32")
+    u0s = [solvers.ic_fourier(s, seed=80 + i) for i, s in enumerate(sizes)]
+    b.set_initial(u0s, tend=10.0, kchunk=kchunk)
+    b.steps(3, graph=True)
+    got = b.solutions()
+    for i, (shape, u0) in enumerate(zip(sizes, u0s)):
+        s = solvers.PeriodicBurgers(shape, scheme="Z2", precision="f32")
+        s.set_initial(u0, tend=10.0)
+        s.steps(3, graph=False)
+        ref = s.solution()
+        assert got[i].dtype == np.float32 and np.array_equal(got[i], ref), (i, shape)
+
+
 def test_batch_serial_path_and_oracle(hipdev):
     b = solvers.BurgersBatch(SIZES[:4], scheme="Z2")
     u0s = [solvers.ic_fourier(s, seed=60 + i) for i, s in enumerate(SIZES[:4])]
