@@ -106,10 +106,12 @@ void Call_RK3d(dim3 numBlocks, dim3 threadsPerBlock, hipStream_t aStream,
  * export two signatures under one name, so they carry a _pitched suffix.
  * Same update sets and arithmetic as the unpitched entry points above;
  * padding columns i >= Nx are neither read nor written.
- *   Burgers: SingleGPU/Burgers3d_WENO5_PitchedMem/Burgers.h:73-77 and
- *   SingleGPU/Burgers3d_WENO5_SharedMem/Burgers.h:79-83.
- *   Heat: SingleGPU/Diffusion3d_PitchedMem/acoustics3d.h:58-59 and
- *   SingleGPU/Diffusion3d_Blocking/acoustics3d.h:59-60. */
+ *   Burgers: SingleGPU/Burgers3d_WENO5_PitchedMem/Burgers.h:73-77; its
+ *   Call_Diff_ and Call_sspRK are also those of Burgers3d_WENO5_SharedMem
+ *   (Burgers.h:82-83, kernels.cu:570-581), whose Call_Adv_* differ (below).
+ *   Heat: SingleGPU/Diffusion3d_PitchedMem/acoustics3d.h:58-59; its Call_RK3d
+ *   is also Diffusion3d_Blocking's (acoustics3d.h:60, kernels.cu:193-198),
+ *   whose Call_Lu3d differs (below). */
 void Call_Adv_x_pitched(dim3 numBlocks, dim3 threadsPerBlock, hipStream_t aStream,
                         unsigned int pitch, unsigned int Nx, unsigned int Ny,
                         unsigned int Nz, double dx, double *u, double *Lu);
@@ -133,6 +135,70 @@ void Call_RK3d_pitched(dim3 numBlocks, dim3 threadsPerBlock, hipStream_t aStream
                        unsigned int step, unsigned int pitch, unsigned int nx,
                        unsigned int ny, unsigned int nz, double dt, double *u,
                        double *uo, double *Lu);
+
+/* ------------------------------------------------------------------------ */
+/* Drivers whose wrappers differ from the base code's                         */
+/* ------------------------------------------------------------------------ */
+/* Replace Call_Adv_x/y/z of SingleGPU/Burgers3d_WENO5_SharedMem
+ * (Burgers.h:79-81; kernels.cu:552-568 launching Compute_dF/dG/dH :212-402
+ * with WENO5Zreconstruction :153-205), pitched arrays.  Unlike the base
+ * sweeps they update every node of the array: the face below node i reads
+ * nodes i-3..i+2 of its line with nodes outside the array taken as 0
+ * (:248-260), and each WENO-Z side is scaled as *(1/6 * (1/sum)) instead of
+ * /(6*sum).  x: Lu = -(F_{i+1/2}-F_{i-1/2})/dx; y, z: Lu -= (..)/d.  Local
+ * LF split; p and eps from mgad_set_adv_scheme (defaults p = 1, eps = 1e-6:
+ * the reference's Burgers.h:26).  Ghost nodes near the high end, where the
+ * reference reads shared memory no thread wrote, get the zero-padded value
+ * (the SSP-RK stage never reads them). */
+void Call_Adv_x_sharedmem(dim3 numBlocks, dim3 threadsPerBlock, hipStream_t aStream,
+                          unsigned int pitch, unsigned int Nx, unsigned int Ny,
+                          unsigned int Nz, double dx, double *u, double *Lu);
+void Call_Adv_y_sharedmem(dim3 numBlocks, dim3 threadsPerBlock, hipStream_t aStream,
+                          unsigned int pitch, unsigned int Nx, unsigned int Ny,
+                          unsigned int Nz, double dy, double *v, double *Lv);
+void Call_Adv_z_sharedmem(dim3 numBlocks, dim3 threadsPerBlock, hipStream_t aStream,
+                          unsigned int pitch, unsigned int Nx, unsigned int Ny,
+                          unsigned int Nz, double dz, double *w, double *Lw);
+/* Replaces Call_Lu3d of SingleGPU/Diffusion3d_Blocking (acoustics3d.h:59;
+ * kernels.cu:187-191 launching Compute_Laplace3d_Async :37-88 with
+ * kstart = 3, kstop = nz-2): Lu = the 13-point operator, each axis summed
+ * from its low side, on 2<i<nx-3, 2<j<ny-3 and planes 3..nz-3 only; other
+ * nodes keep their Lu (the driver zeroes Lu once, main.cpp:101, so the
+ * ring i = 2 stays at its initial value under Call_RK3d_pitched).  Planes
+ * the reference also writes when nz % 16 is 1 or 2 (a block's unconditional
+ * first plane, reading past the array) are not written. */
+void Call_Lu3d_blocking(dim3 numBlocks, dim3 threadsPerBlock, hipStream_t aStream,
+                        unsigned int pitch, unsigned int nx, unsigned int ny,
+                        unsigned int nz, double *u, double *Lu);
+/* Replace Call_Diff_ and Call_sspRK of MultiGPU/Diffusion3d_Baseline
+ * (DiffusionMPICUDA.h:118-123, Kernels.cu:401-414), whose signatures clash
+ * with the single-GPU ones.  One rank's slab of _Nz planes (ghost planes
+ * included), row pitch `pitch`:
+ *   Call_Diff_mpi: LaplaceO4_async (Kernels.cu:207-260), the kernel the
+ *     wrapper launches: Lu = diff_x/12 * (one left-to-right sum of the 12
+ *     neighbours and -90 u) on 2<i<Nx-3, 2<j<Ny-3, planes kstart..kstop-1
+ *     (2 <= kstart <= kstop <= _Nz-2; diff_y, diff_z unused, as there);
+ *   Call_Diff_O2_mpi: LaplaceO2_async (Kernels.cu:152-201), the wrapper's
+ *     commented-out alternative: Lu = diff_x (u_w - 2u + u_e) + diff_y (..)
+ *     + diff_z (..), 1 <= kstart <= kstop <= _Nz-1;
+ *   Call_sspRK_mpi: Compute_RK (Kernels.cu:266-297) on 2<i<Nx-3, 2<j<Ny-3
+ *     and every plane of the slab.
+ * The planes written are exactly kstart..kstop-1 (the reference's blocks
+ * of 16 planes also write their first plane when it lies past kstop). */
+void Call_Diff_mpi(dim3 numBlocks, dim3 threadsPerBlock, hipStream_t aStream,
+                   unsigned int pitch, unsigned int Nx, unsigned int Ny,
+                   unsigned int _Nz, unsigned int kstart, unsigned int kstop,
+                   double diff_x, double diff_y, double diff_z, double *q,
+                   double *Lq);
+void Call_Diff_O2_mpi(dim3 numBlocks, dim3 threadsPerBlock, hipStream_t aStream,
+                      unsigned int pitch, unsigned int Nx, unsigned int Ny,
+                      unsigned int _Nz, unsigned int kstart, unsigned int kstop,
+                      double diff_x, double diff_y, double diff_z, double *q,
+                      double *Lq);
+void Call_sspRK_mpi(dim3 numBlocks, dim3 threadsPerBlock, hipStream_t aStream,
+                    unsigned int step, unsigned int pitch, unsigned int Nx,
+                    unsigned int Ny, unsigned int _Nz, double dt, double *q,
+                    double *qo, double *Lq);
 
 /* ------------------------------------------------------------------------ */
 /* MultiGPU/Burgers3d_Baseline device wrappers: one rank's z slab            */
@@ -243,7 +309,10 @@ int mgad_burgers_init(hipStream_t stream, unsigned int nx, unsigned int ny,
  *   stage 3: u_out = (u_n + 2 (u_in + dt*L(u_in)))/3
  * with L(u) = -dF/dx - dG/dy - dH/dz (WENO5 face fluxes, the order of
  * Call_Adv_x -> Call_Adv_y -> Call_Adv_z).  Stage 1 sets dt =
- * cfl*dx/max|u_n| clipped to tend; the step is a no-op once t >= tend.
+ * cfl*dx/max|u_n| clipped to tend; the step is a no-op once t >= tend, and
+ * the run stops (dt = 0; t, the step count and u as after the last completed
+ * step) when a stage input's max|u| is NaN or infinite, which
+ * mgad_burgers_read_ctrl then reports as MGAD_ERR_NONFINITE.  Any nx, ny, nz >= 4 with fewer than 2^31 points.
  * u_out may alias u_n (stage 3 runs in place), never u_in. */
 int mgad_burgers_stage(hipStream_t stream, const mgad_scheme *scheme,
                        unsigned int nx, unsigned int ny, unsigned int nz,
@@ -352,6 +421,7 @@ int mgad_batch_steps(hipStream_t stream, const mgad_batch *batch, int nsteps,
 int mgad_batch_steps_serial(hipStream_t stream, const mgad_batch *batch,
                             const mgad_problem *probs, int nsteps,
                             int use_graph);
+
 
 /* ------------------------------------------------------------------------ */
 /* z-slab decomposition of the periodic Burgers run (FP64)                   */

@@ -25,6 +25,8 @@ EXPORTS = (
     "Call_sspRK", "Call_Lu3d", "Call_RK3d",
     "Call_Adv_x_pitched", "Call_Adv_y_pitched", "Call_Adv_z_pitched", "Call_Diff_pitched", "Call_sspRK_pitched",
     "Call_Lu3d_pitched", "Call_RK3d_pitched",
+    "Call_Adv_x_sharedmem", "Call_Adv_y_sharedmem", "Call_Adv_z_sharedmem", "Call_Lu3d_blocking",
+    "Call_Diff_mpi", "Call_Diff_O2_mpi", "Call_sspRK_mpi",
     "mgad_set_adv_scheme", "mgad_lu3d_o2",
     "mgad_burgers_init", "mgad_burgers_stage", "mgad_burgers_steps", "mgad_burgers_read_ctrl",
     "mgad_burgers_init_p", "mgad_burgers_steps_p", "mgad_burgers_autotune", "mgad_burgers_set_launch",
@@ -115,6 +117,18 @@ def lib():
         L.Call_Lu3d_pitched.restype = None
         L.Call_RK3d_pitched.argtypes = [Dim3, Dim3, VP, U, U, U, U, U, D, VP, VP, VP]
         L.Call_RK3d_pitched.restype = None
+        for name in ("Call_Adv_x_sharedmem", "Call_Adv_y_sharedmem", "Call_Adv_z_sharedmem"):
+            f = getattr(L, name)
+            f.argtypes = [Dim3, Dim3, VP, U, U, U, U, D, VP, VP]
+            f.restype = None
+        L.Call_Lu3d_blocking.argtypes = [Dim3, Dim3, VP, U, U, U, U, VP, VP]
+        L.Call_Lu3d_blocking.restype = None
+        for name in ("Call_Diff_mpi", "Call_Diff_O2_mpi"):
+            f = getattr(L, name)
+            f.argtypes = [Dim3, Dim3, VP, U, U, U, U, U, U, D, D, D, VP, VP]
+            f.restype = None
+        L.Call_sspRK_mpi.argtypes = [Dim3, Dim3, VP, U, U, U, U, U, D, VP, VP, VP]
+        L.Call_sspRK_mpi.restype = None
         for name in ("Compute_Adv_x_Async", "Compute_Adv_y_Async", "Compute_Adv_z_Async"):
             f = getattr(L, name)
             f.argtypes = [Dim3, Dim3, VP, U, U, U, U, U, U, D, VP, VP]

@@ -8,6 +8,7 @@ FP32 instructions disabled.
 """
 from __future__ import annotations
 
+import concurrent.futures
 import os
 import shutil
 import subprocess
@@ -18,7 +19,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libmgad.so")
 ARCH = os.environ.get("MGAD_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["burgers_periodic.hip", "burgers_ref.hip", "batch.hip", "adjoint.hip", "dropin.hip", "heat.hip", "selfcheck.hip", "slab.hip", "weno7.hip",
+SOURCES = ["burgers_periodic.hip", "burgers_ref.hip", "batch.hip", "adjoint.hip", "dropin.hip", "heat.hip", "selfcheck.hip", "slab.hip", "variants.hip", "weno7.hip",
            "host.cpp", "io.cpp"]
 # Per-file changes to HIPCC_FLAGS: the adjoint is checked to a tolerance, not
 # bit for bit, so it lets the compiler fuse multiply-adds.
@@ -47,15 +48,20 @@ def build(verbose: bool = False) -> str:
     """Compile every source and link libmgad.so; returns its path."""
     objdir = os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)
-    objs = []
+    objs, cmds = [], []
     for src in SOURCES:
         obj = os.path.join(objdir, os.path.splitext(src)[0] + ".o")
         flags = [FILE_FLAGS.get(src, {}).get(f, f) for f in HIPCC_FLAGS]
-        cmd = [hipcc(), *flags, "-c", os.path.join(CSRC, src), "-o", obj]
-        if verbose:
-            print(" ".join(cmd), flush=True)
-        subprocess.run(cmd, check=True)
+        cmds.append([hipcc(), *flags, "-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
+    # the sources compile independently: one hipcc per source, a few at a time
+    jobs = max(1, min(len(cmds), int(os.environ.get("MAX_JOBS", "0")) or (os.cpu_count() or 1), 8))
+    with concurrent.futures.ThreadPoolExecutor(jobs) as pool:
+        for cmd in cmds:
+            if verbose:
+                print(" ".join(cmd), flush=True)
+        for fut in [pool.submit(subprocess.run, cmd, check=True) for cmd in cmds]:
+            fut.result()
     tmp = LIB + ".tmp"
     cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp]
     if verbose:

@@ -34,7 +34,9 @@
 
 namespace mgad {
 
-template <class T, int WENO, int P, int TX, int TY>
+// XW: nx or ny is not a multiple of the tile; the last tile of each row or
+// column reaches past the grid and stores only its real nodes.
+template <class T, int WENO, int P, int TX, int TY, bool XW>
 __global__ __launch_bounds__(TX * TY, 4) void k_burgers_stage(const T *__restrict__ uin, const T *un,
                                                            T *uout, GridArgs g, double eps, int stage,
                                                            Ctrl *ctrl) {
@@ -44,11 +46,11 @@ __global__ __launch_bounds__(TX * TY, 4) void k_burgers_stage(const T *__restric
     if (!stage_control(ctrl, stage, g.dx, lead, dt, alpha)) return;
     const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
     if (fast_division_ok<WENO>(alpha, eps))
-        stage_tile<T, WENO, P, TX, TY, true>(uin, un, uout, g, stage, tile, dt, alpha, eps, sm,
-                                             &ctrl->amax[stage % 3]);
+        stage_tile<T, WENO, P, TX, TY, true, false, XW>(uin, un, uout, g, stage, tile, dt, alpha, eps, sm,
+                                                        &ctrl->amax[stage % 3]);
     else
-        stage_tile<T, WENO, P, TX, TY, false>(uin, un, uout, g, stage, tile, dt, alpha, eps, sm,
-                                              &ctrl->amax[stage % 3]);
+        stage_tile<T, WENO, P, TX, TY, false, false, XW>(uin, un, uout, g, stage, tile, dt, alpha, eps, sm,
+                                                         &ctrl->amax[stage % 3]);
 }
 
 // max|u| over n values into *slot (atomicMax of the bit pattern).
@@ -98,17 +100,24 @@ struct Shape {
     StageKernel<T> fn;
 };
 
+// Tiles that divide the grid when they can: 32x16 (unless 16 is asked for),
+// else 16x16; otherwise tiles that reach past the grid, 32x16 for rows of 64
+// nodes or more (or when 32 is asked for), else 16x16.
 template <class T, int WENO, int P>
 bool pick(int nx, int ny, int tile_pref, Shape<T> &s) {
     if (tile_pref != 16 && nx % 32 == 0 && ny % 16 == 0) {
-        s = {32, 16, k_burgers_stage<T, WENO, P, 32, 16>};
+        s = {32, 16, k_burgers_stage<T, WENO, P, 32, 16, false>};
         return true;
     }
     if (nx % 16 == 0 && ny % 16 == 0) {
-        s = {16, 16, k_burgers_stage<T, WENO, P, 16, 16>};
+        s = {16, 16, k_burgers_stage<T, WENO, P, 16, 16, false>};
         return true;
     }
-    return false;
+    if (tile_pref == 32 || (tile_pref != 16 && nx >= 64))
+        s = {32, 16, k_burgers_stage<T, WENO, P, 32, 16, true>};
+    else
+        s = {16, 16, k_burgers_stage<T, WENO, P, 16, 16, true>};
+    return true;
 }
 
 template <class T>
@@ -124,14 +133,15 @@ bool pick_kernel(const mgad_scheme &sc, int nx, int ny, int tile_pref, Shape<T> 
 int default_kchunk(int nx, int ny, int nz, int tx, int ty) {
     // Enough blocks to give every CU several tiles; chunks of >= 8 planes so
     // the 6-plane queue warm-up stays small.
-    const long tiles = static_cast<long>(nx / tx) * (ny / ty);
+    const long tiles = static_cast<long>((nx + tx - 1) / tx) * ((ny + ty - 1) / ty);
     int kc = nz;
     while (kc > 8 && tiles * ((nz + kc - 1) / kc) < 2048) kc = (kc + 1) / 2;
     return kc;
 }
 
 // The z queue wraps plane indices by at most one period (z0-3 and k+4), so
-// nz >= 4; x and y tiles divide the extents, and indices stay below 2^31.
+// nz >= 4; x and y halo indices wrap from -3, so nx, ny >= 4 as well; indices
+// stay below 2^31.
 bool valid_grid(unsigned nx, unsigned ny, unsigned nz) {
     if (nx < 4 || ny < 4 || nz < 4) return false;
     const unsigned long long n = 1ull * nx * ny * nz;
@@ -168,7 +178,7 @@ int burgers_stage_launch_t(hipStream_t stream, const mgad_scheme &sc, unsigned n
         return MGAD_ERR_UNSUPPORTED;
     }
     if (kchunk <= 0) kchunk = default_kchunk(nx, ny, nz, s.tx, s.ty);
-    const unsigned nblocks = (nx / s.tx) * (ny / s.ty) * ((nz + kchunk - 1) / kchunk);
+    const unsigned nblocks = ((nx + s.tx - 1) / s.tx) * ((ny + s.ty - 1) / s.ty) * ((nz + kchunk - 1) / kchunk);
     const GridArgs g = make_grid(nx, ny, nz, dx, dy, dz, kchunk);
     hipLaunchKernelGGL(s.fn, dim3(nblocks), dim3(s.tx * s.ty), 0, stream, u_in, u_n, u_out, g, sc.eps,
                        stage, static_cast<Ctrl *>(ctrl));
@@ -256,8 +266,8 @@ int autotune_t(hipStream_t stream, const mgad_scheme &sc, unsigned nx, unsigned 
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if ((st = check_hip(hipEventCreate(&e0))) || (st = check_hip(hipEventCreate(&e1)))) return st;
     std::vector<int> tiles;
-    if (nx % 32 == 0 && ny % 16 == 0) tiles.push_back(32);
-    if (nx % 16 == 0 && ny % 16 == 0) tiles.push_back(16);
+    if (nx % 32 == 0 || (nx % 16 != 0 && nx >= 32) || ny % 16 != 0) tiles.push_back(32);
+    tiles.push_back(16);
     std::vector<int> chunks;
     for (int kc : {4, 8, 16, 32, 64, 128})
         if (kc <= static_cast<int>(nz)) chunks.push_back(kc);
@@ -450,13 +460,15 @@ extern "C" int mgad_burgers_read_ctrl(hipStream_t stream, const void *ctrl, mgad
     out->dt = c.dt;
     out->tend = c.tend;
     out->cfl = c.cfl;
-    double a;
-    const unsigned long long b = c.amax[0];
-    static_assert(sizeof(a) == sizeof(b), "double size");
-    __builtin_memcpy(&a, &b, sizeof(a));
-    out->amax = a;
+    double a[3];
+    static_assert(sizeof(a) == sizeof(c.amax), "double size");
+    __builtin_memcpy(a, c.amax, sizeof(a));
+    out->amax = a[0];
     out->steps = c.steps;
-    if (!(a <= 1.7976931348623157e308)) {
+    // a stage that found its input's max|u| NaN or infinite left it in its
+    // slot and stopped the run (dt = 0)
+    if (!(a[0] <= 1.7976931348623157e308) || !(a[1] <= 1.7976931348623157e308) ||
+        !(a[2] <= 1.7976931348623157e308)) {
         set_error(MGAD_ERR_NONFINITE);
         return MGAD_ERR_NONFINITE;
     }

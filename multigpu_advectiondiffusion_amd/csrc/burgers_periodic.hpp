@@ -43,16 +43,23 @@ __device__ __forceinline__ int wrapi(int i, int n) { return i < 0 ? i + n : (i >
 // max|u^n| (all blocks compute the same value) and records it; stage 2 moves
 // t forward.  Each stage's lead thread clears the max|u| slot the next-but-one
 // stage accumulates into, and fills the step record when one is kept.
-// Returns false once the run has reached tend.
+// Returns false once the run has reached tend, or when a stage input's
+// max|u| is NaN or infinite: then dt is set to 0, so the rest of the step is
+// skipped too (u^n stays as it was; only stage 3 writes it), t and the step
+// count stay, and the non-finite max|u| is left in its slot for the host to
+// report.  Stage 1 also checks the slots of stages 2 and 3, so a run stopped
+// inside a step stays stopped.
 __device__ __forceinline__ bool stage_control(Ctrl *ctrl, int stage, double dx, bool lead, double &dt,
                                               double &alpha) {
     if (stage == 1) {
         const double t = ctrl->t, tend = ctrl->tend;
-        if (!(t < tend)) {
+        const double a = bits_double(ctrl->amax[0]);
+        const unsigned long long inf_bits = 0x7FF0000000000000ull;
+        if (!(t < tend) || !(a <= 1.7976931348623157e308) || ctrl->amax[1] >= inf_bits ||
+            ctrl->amax[2] >= inf_bits) {
             if (lead) ctrl->dt = 0.0;
             return false;
         }
-        const double a = bits_double(ctrl->amax[0]);
         dt = ctrl->cfl * dx / a;
         if (t + dt > tend) dt = tend - t;
         alpha = a;
@@ -71,6 +78,10 @@ __device__ __forceinline__ bool stage_control(Ctrl *ctrl, int stage, double dx, 
     dt = ctrl->dt;
     if (dt == 0.0) return false;
     alpha = bits_double(ctrl->amax[stage - 1]);
+    if (!(alpha <= 1.7976931348623157e308)) {
+        if (lead) ctrl->dt = 0.0;
+        return false;
+    }
     if (lead) {
         if (stage == 2) {
             if (ctrl->rec) ctrl->rec[ctrl->steps - ctrl->rec_step0].a2 = alpha;
@@ -108,10 +119,11 @@ struct TileSmem {
 // ZG, planes 0..2 are also stored into the top ghost planes of the slab set
 // at gh_lo and planes nz-3..nz-1 into the bottom ghost planes at gh_hi (the
 // neighbours' copies of them, so no separate exchange is needed).
-// XW: the tile may reach past nx (x0 + TX > nx); its columns x >= nx stand
-// for x mod nx and are computed like the others, so every face of the real
-// columns is there, but are neither stored nor counted in max|u| (in stage 3
-// their u^n may already hold the owner's new value).
+// XW: the tile may reach past nx or ny (x0 + TX > nx, y0 + TY > ny); its
+// columns x >= nx stand for x mod nx and its rows y >= ny for y mod ny.  They
+// are computed like the others, so every face of the real columns is there,
+// but are neither stored nor counted in max|u| (in stage 3 their u^n may
+// already hold the owner's new value).
 template <class T, int WENO, int P, int TX, int TY, bool FD, bool ZG = false, bool XW = false>
 __device__ __forceinline__ void stage_cols(const T *__restrict__ uin, const T *un, T *uout,
                                            const GridArgs &g, int stage, int x0, int y0, int z0, int z1,
@@ -128,12 +140,13 @@ __device__ __forceinline__ void stage_cols(const T *__restrict__ uin, const T *u
     const int tx = tid % TX, ty = tid / TX;
     const int wave = tid >> 6, lane = tid & 63;
     const unsigned plane = static_cast<unsigned>(nx) * static_cast<unsigned>(ny);
-    // x index of a column or halo node at x0 + i (i >= -3): wrapped once by
-    // periodicity, or reduced mod nx when the tile reaches past nx
+    // x (y) index of a column (row) or halo node at x0 + i (i >= -3): wrapped
+    // once by periodicity, or reduced mod nx (ny) when the tile reaches past it
     auto xmod = [&](int i) { return XW ? (i + nx) % nx : wrapi(i, nx); };
-    const bool own_col = !XW || x0 + tx < nx;
-    const unsigned col =
-        static_cast<unsigned>(XW ? xmod(x0 + tx) : x0 + tx) + static_cast<unsigned>(nx) * (y0 + ty);
+    auto ymod = [&](int i) { return XW ? (i + ny) % ny : wrapi(i, ny); };
+    const bool own_col = !XW || (x0 + tx < nx && y0 + ty < ny);
+    const unsigned col = static_cast<unsigned>(XW ? xmod(x0 + tx) : x0 + tx) +
+                         static_cast<unsigned>(nx) * static_cast<unsigned>(XW ? ymod(y0 + ty) : y0 + ty);
     // storage plane of plane k: zw for k in -3..nz+3 (wrapped when periodic), zi for 0 <= k < nz
     auto zw = [&](int k) { return static_cast<unsigned>(ZG ? k + 3 : wrapi(k, nz)); };
     auto zi = [&](int k) { return static_cast<unsigned>(ZG ? k + 3 : k); };
@@ -146,14 +159,15 @@ __device__ __forceinline__ void stage_cols(const T *__restrict__ uin, const T *u
         const int r = tid / 6, c = tid % 6;
         hl_r = r + 3;
         hl_c = c < 3 ? c : TX + c;
-        hcol = static_cast<unsigned>(xmod(x0 + hl_c - 3)) + static_cast<unsigned>(nx) * (y0 + r);
+        hcol = static_cast<unsigned>(xmod(x0 + hl_c - 3)) +
+               static_cast<unsigned>(nx) * static_cast<unsigned>(XW ? ymod(y0 + r) : y0 + r);
     } else if (has_halo) {
         const int h = tid - 6 * TY;
         const int c = h % TX, q = h / TX;
         hl_r = q < 3 ? q : TY + q;
         hl_c = c + 3;
         hcol = static_cast<unsigned>(XW ? xmod(x0 + c) : x0 + c) +
-               static_cast<unsigned>(nx) * wrapi(y0 + hl_r - 3, ny);
+               static_cast<unsigned>(nx) * static_cast<unsigned>(ymod(y0 + hl_r - 3));
     }
 
     // z register rings (see the march below): qp/qm hold the split fluxes of
@@ -310,20 +324,22 @@ __device__ __forceinline__ void stage_cols(const T *__restrict__ uin, const T *u
 }
 
 // stage_cols for block `tile` of the chunked decomposition: tiles x-fastest,
-// then y, then z chunks of g.kchunk planes.
-template <class T, int WENO, int P, int TX, int TY, bool FD, bool ZG = false>
+// then y, then z chunks of g.kchunk planes.  XW: the last tile of a row or
+// column may reach past nx or ny (see stage_cols).
+template <class T, int WENO, int P, int TX, int TY, bool FD, bool ZG = false, bool XW = false>
 __device__ __forceinline__ void stage_tile(const T *__restrict__ uin, const T *un, T *uout,
                                            const GridArgs &g, int stage, unsigned tile, double dt,
                                            double alpha, double eps, TileSmem<TX, TY> &sm,
                                            unsigned long long *amax_slot, T *gh_lo = nullptr,
                                            T *gh_hi = nullptr) {
-    const int ntx = g.nx / TX, nty = g.ny / TY;
+    const int ntx = XW ? (g.nx + TX - 1) / TX : g.nx / TX, nty = XW ? (g.ny + TY - 1) / TY : g.ny / TY;
     const int bx = tile % ntx;
     const int by = (tile / ntx) % nty;
     const int bz = tile / (ntx * nty);
     const int z0 = bz * g.kchunk;
-    stage_cols<T, WENO, P, TX, TY, FD, ZG>(uin, un, uout, g, stage, bx * TX, by * TY, z0, min(z0 + g.kchunk, g.nz),
-                                           dt, alpha, eps, sm, amax_slot, gh_lo, gh_hi);
+    stage_cols<T, WENO, P, TX, TY, FD, ZG, XW>(uin, un, uout, g, stage, bx * TX, by * TY, z0,
+                                               min(z0 + g.kchunk, g.nz), dt, alpha, eps, sm, amax_slot, gh_lo,
+                                               gh_hi);
 }
 
 int default_kchunk(int nx, int ny, int nz, int tx, int ty);

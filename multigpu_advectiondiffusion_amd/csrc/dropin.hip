@@ -34,6 +34,8 @@ constexpr int kChunk = 32;  // faces per thread and line chunk
 
 }  // namespace
 
+AdvParams adv_params() { return AdvParams{g_adv.weno, g_adv.p, g_adv.eps, g_adv.lf, g_adv.alpha}; }
+
 // Which nodes a sweep updates: lines over the first transverse coordinate
 // a = 3..n_a-4 and the second b = b0..b0+nb-1; along each line nodes l0..l1-1,
 // using faces l0-1/2 .. l1-1/2.  zero_first: the face below node l0 is 0 (the
@@ -484,8 +486,10 @@ void halo_copy(hipStream_t s, double *q, double *buf, unsigned nx, unsigned ny, 
 using namespace mgad;
 
 extern "C" int mgad_set_adv_scheme(int weno, int p, double eps, int lf, double alpha) {
+    // a global LF alpha bounds |f'(u)| = |u| from above: finite and >= 0
     if ((weno != MGAD_WENO_Z && weno != MGAD_WENO_JS) || (p != 1 && p != 2) ||
-        (lf != MGAD_LF_LOCAL && lf != MGAD_LF_GLOBAL) || !(eps >= 0.0)) {
+        (lf != MGAD_LF_LOCAL && lf != MGAD_LF_GLOBAL) || !(eps >= 0.0) ||
+        !(alpha >= 0.0 && alpha <= 1.7976931348623157e308)) {
         set_error(MGAD_ERR_INVALID_ARG);
         return MGAD_ERR_INVALID_ARG;
     }

@@ -167,9 +167,10 @@ __global__ __launch_bounds__(256) void k_heat7_fe(const double *__restrict__ u, 
 }
 
 // What k_heat13_rk writes: an SSP-RK3 stage of the heat step, the 13-point
-// operator itself (Call_Lu3d), or the operator added to Lu over the Burgers
-// viscous region (Call_Diff_).
-enum : int { kHeatRk = 0, kHeatLu = 1, kHeatDiffB5 = 2 };
+// operator itself (Call_Lu3d), the operator added to Lu over the Burgers
+// viscous region (Call_Diff_), or stored there (the blocked heat code's
+// Call_Lu3d).
+enum : int { kHeatRk = 0, kHeatLu = 1, kHeatDiffB5 = 2, kHeatLuB5 = 3 };
 
 // One SSP-RK3 stage of the reference heat step fused: the 13-point Laplacian
 // of Compute_Laplace3d (Diffusion3d_baselineCode/kernels.cu:94-97, with the
@@ -181,13 +182,15 @@ enum : int { kHeatRk = 0, kHeatLu = 1, kHeatDiffB5 = 2 };
 // (SingleGPU/Burgers3d_WENO5/kernels.cu:444-494): nodes 3..n-4 in x and y,
 // planes 3..nz-3, each axis's 4th-order difference summed from the low side
 // (-u_{-2} + 16 u_{-1} ... instead of from the high side), added to Lu,
-// which the un ring then reads (un = uout = Lu).
+// which the un ring then reads (un = uout = Lu).  MODE kHeatLuB5 is
+// Compute_Laplace3d_Async of SingleGPU/Diffusion3d_Blocking (kernels.cu:
+// 37-88, Call_Lu3d :187-191): the same region and sums, stored into Lu.
 template <int TY, int D, int MODE = kHeatRk>
 __global__ __launch_bounds__(256) void k_heat13_rk(const double *__restrict__ uin, const double *un, double *uout,
                                                    int nx, int ny, int nz, int px, int ntx, int nty, int kchunk,
                                                    double kx, double ky, double kz, double dt, int stage) {
     static_assert(D >= 1 && D <= 3, "prefetch distance");
-    constexpr bool LU = MODE == kHeatLu, B5 = MODE == kHeatDiffB5;
+    constexpr bool LU = MODE == kHeatLu || MODE == kHeatLuB5, B5 = MODE == kHeatDiffB5 || MODE == kHeatLuB5;
     constexpr int LO = B5 ? 3 : 2;  // first interior node of x and y (and z)
     constexpr int RPT = TY / 4, W = kHeatTX + 4;
     __shared__ double sp[2][TY + 4][W];
@@ -269,7 +272,7 @@ __global__ __launch_bounds__(256) void k_heat13_rk(const double *__restrict__ ui
             const double Lu = (kx * X + ky * Y) + kz * Z;
             if (LU) {
                 pv.store(so[r], Lu);
-            } else if (B5) {
+            } else if (B5) {  // kHeatDiffB5
                 pv.store(so[r], o[r][s & 3] + Lu);
             } else {
                 const double uo = stage == 1 ? c : o[r][s & 3];
@@ -411,6 +414,20 @@ int diff13_b5_launch(hipStream_t s, unsigned nx, unsigned ny, unsigned nz, unsig
     const auto kern = k_heat13_rk<kHeat13TY, kHeat13D, kHeatDiffB5>;
     hipLaunchKernelGGL(kern, dim3(g.blocks), dim3(256), 0, s, u, Lu, Lu, (int)nx, (int)ny, (int)nz, (int)px, g.ntx,
                        g.nty, g.kc, kx, ky, kz, 0.0, 1);
+    return check_launch();
+}
+
+// Compute_Laplace3d_Async of SingleGPU/Diffusion3d_Blocking (kernels.cu:
+// 37-88, launched by Call_Lu3d :187-191 with kstart 3, kstop nz-2): Lu = the
+// 13-point operator on nodes 3..n-4 in x and y and planes 3..nz-3, summed
+// as Call_Diff_'s operator; row pitch px, planes of px*ny < 2^29 values,
+// nx, ny >= 7, nz >= 6.
+int lu13_b5_launch(hipStream_t s, unsigned nx, unsigned ny, unsigned nz, unsigned px, const double *u, double *Lu) {
+    const HeatGrid g = heat13_grid(nx - 6, ny - 6, nz - 5);
+    const DiffCoef c = diff_coef();
+    const auto kern = k_heat13_rk<kHeat13TY, kHeat13D, kHeatLuB5>;
+    hipLaunchKernelGGL(kern, dim3(g.blocks), dim3(256), 0, s, u, u, Lu, (int)nx, (int)ny, (int)nz, (int)px, g.ntx,
+                       g.nty, g.kc, c.kx, c.ky, c.kz, 0.0, 1);
     return check_launch();
 }
 

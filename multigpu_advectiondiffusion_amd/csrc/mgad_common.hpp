@@ -28,6 +28,15 @@ int check_launch();                    // hipGetLastError after a launch
 // the launch table, which takes it again.
 std::recursive_mutex &host_mutex();
 
+// The scheme of the Call_Adv_* sweeps (mgad_set_adv_scheme).
+struct AdvParams {
+    int weno, p;
+    double eps;
+    int lf;
+    double alpha;
+};
+AdvParams adv_params();
+
 // Diffusion coefficients of CopyToConstantMemory, passed to kernels by value.
 struct DiffCoef { double kx, ky, kz; };
 DiffCoef diff_coef();
@@ -36,6 +45,7 @@ int heat13_lu_launch(hipStream_t s, unsigned nx, unsigned ny, unsigned nz, unsig
                      double *Lu);
 int diff13_b5_launch(hipStream_t s, unsigned nx, unsigned ny, unsigned nz, unsigned px, const double *u, double *Lu,
                      double kx, double ky, double kz);
+int lu13_b5_launch(hipStream_t s, unsigned nx, unsigned ny, unsigned nz, unsigned px, const double *u, double *Lu);
 
 // ---------------------------------------------------------------------------
 // Device: WENO5 arithmetic.
@@ -45,8 +55,12 @@ int diff13_b5_launch(hipStream_t s, unsigned nx, unsigned ny, unsigned nz, unsig
 constexpr double kD0N = 1. / 10., kD1N = 6. / 10., kD2N = 3. / 10.;
 constexpr double kD0P = 3. / 10., kD1P = 6. / 10., kD2P = 1. / 10.;
 constexpr double kC1312 = 13. / 12., kC14 = 1. / 4.;
+constexpr double kC1o6 = 1. / 6.;  // SingleGPU/Burgers3d_WENO5_SharedMem/Burgers.h:30
 
-enum : int { kWenoZ = 0, kWenoJS = 1 };
+// kWenoZB5S: WENO-Z with the shared-memory code's rounding of the final
+// weight, *(1/6 * (1/sum)) (SingleGPU/Burgers3d_WENO5_SharedMem/
+// kernels.cu:175-178, :198-201) instead of /(6*sum).
+enum : int { kWenoZ = 0, kWenoJS = 1, kWenoZB5S = 2 };
 
 // Correctly rounded a/b for a divisor fixed on the host, given rb = RN(1/b)
 // computed there: q = RN(a*rb), rem = a - b*q (exact with an FMA), and
@@ -116,7 +130,8 @@ __device__ __forceinline__ double div_inrange(double a, double b) {
 
 // When the weight divisions of weno_side_s may use div_inrange (its FD
 // variant) with results identical to IEEE division.  With max|u| <= 2^40 over
-// the stage input every split flux has |f| < 2^80, so each 13/12 s^2 < 2^165
+// the stage input (and |alpha| <= 2^40 for a global LF split, checked here;
+// callers check |u|) every split flux has |f| < 2^80, so each 13/12 s^2 < 2^165
 // and 1/4 q^2 < 2^164: 0 <= beta_k < 2^166.
 //   WENO-Z, 2^-300 <= eps <= 1: b = beta+eps is in [2^-300, 2^167] and
 //     a = tau5 < 2^167.  For tau5 = 0 or tau5 >= 2^-380 the ranges above
@@ -130,8 +145,8 @@ __device__ __forceinline__ double div_inrange(double a, double b) {
 // or infinite alpha fails the test.
 template <int WENO>
 __host__ __device__ inline bool fast_division_ok(double alpha, double eps) {
-    const double eps_lo = WENO == kWenoZ ? 0x1p-300 : 0x1p-190;
-    return alpha <= 0x1p40 && eps >= eps_lo && eps <= 1.0;
+    const double eps_lo = WENO == kWenoJS ? 0x1p-190 : 0x1p-300;
+    return fabs(alpha) <= 0x1p40 && eps >= eps_lo && eps <= 1.0;
 }
 
 // WENO-Z weight d*(1 + (tau5/(beta+eps))^P), kernels.cu:163-165 (P = 1);
@@ -171,12 +186,13 @@ __device__ __forceinline__ double weno_side_s(double v0, double v1, double v2, d
         c2 = p2fma(2.0, v4, 11.0 * v2 - 7.0 * v3);     // (11v2 - 7v3) + 2v4
         d0 = kD0P; d1 = kD1P; d2 = kD2P;
     }
-    if (WENO == kWenoZ) {
+    if (WENO == kWenoZ || WENO == kWenoZB5S) {
         const double tau5 = fabs(B0 - B2);
         const double a0 = zweight<P, FD>(d0, tau5, B0, eps);
         const double a1 = zweight<P, FD>(d1, tau5, B1, eps);
         const double a2 = zweight<P, FD>(d2, tau5, B2, eps);
         const double sum = (a0 + a1) + a2;
+        if (WENO == kWenoZB5S) return ((a0 * c0 + a1 * c1) + a2 * c2) * (kC1o6 * (1.0 / sum));
         return ((a0 * c0 + a1 * c1) + a2 * c2) / (6.0 * sum);
     } else {
         const double e0 = (eps + B0) * (eps + B0);

@@ -131,6 +131,51 @@ def Call_RK3d_pitched(numBlocks, threadsPerBlock, aStream, step, pitch, nx, ny, 
           dt, _ptr(u), _ptr(uo), _ptr(Lu))
 
 
+# SingleGPU/Burgers3d_WENO5_SharedMem (Burgers.h:79-81): sweeps over every
+# node with a zero pad beyond the array and the *(1/6 * (1/sum)) rounding.
+
+def Call_Adv_x_sharedmem(numBlocks, threadsPerBlock, aStream, pitch, Nx, Ny, Nz, dx, u, Lu) -> None:
+    _call("Call_Adv_x_sharedmem", _dim(numBlocks), _dim(threadsPerBlock), _stream(aStream), pitch, Nx, Ny, Nz, dx,
+          _ptr(u), _ptr(Lu))
+
+
+def Call_Adv_y_sharedmem(numBlocks, threadsPerBlock, aStream, pitch, Nx, Ny, Nz, dy, v, Lv) -> None:
+    _call("Call_Adv_y_sharedmem", _dim(numBlocks), _dim(threadsPerBlock), _stream(aStream), pitch, Nx, Ny, Nz, dy,
+          _ptr(v), _ptr(Lv))
+
+
+def Call_Adv_z_sharedmem(numBlocks, threadsPerBlock, aStream, pitch, Nx, Ny, Nz, dz, w, Lw) -> None:
+    _call("Call_Adv_z_sharedmem", _dim(numBlocks), _dim(threadsPerBlock), _stream(aStream), pitch, Nx, Ny, Nz, dz,
+          _ptr(w), _ptr(Lw))
+
+
+# SingleGPU/Diffusion3d_Blocking (acoustics3d.h:59): Lu on nodes 3..n-4,
+# planes 3..nz-3, each axis summed from its low side.
+
+def Call_Lu3d_blocking(numBlocks, threadsPerBlock, aStream, pitch, nx, ny, nz, u, Lu) -> None:
+    _call("Call_Lu3d_blocking", _dim(numBlocks), _dim(threadsPerBlock), _stream(aStream), pitch, nx, ny, nz,
+          _ptr(u), _ptr(Lu))
+
+
+# MultiGPU/Diffusion3d_Baseline (DiffusionMPICUDA.h:118-123): one rank's slab.
+
+def Call_Diff_mpi(numBlocks, threadsPerBlock, aStream, pitch, Nx, Ny, _Nz, kstart, kstop, diff_x, diff_y, diff_z,
+                  q, Lq) -> None:
+    _call("Call_Diff_mpi", _dim(numBlocks), _dim(threadsPerBlock), _stream(aStream), pitch, Nx, Ny, _Nz, kstart,
+          kstop, diff_x, diff_y, diff_z, _ptr(q), _ptr(Lq))
+
+
+def Call_Diff_O2_mpi(numBlocks, threadsPerBlock, aStream, pitch, Nx, Ny, _Nz, kstart, kstop, diff_x, diff_y,
+                     diff_z, q, Lq) -> None:
+    _call("Call_Diff_O2_mpi", _dim(numBlocks), _dim(threadsPerBlock), _stream(aStream), pitch, Nx, Ny, _Nz, kstart,
+          kstop, diff_x, diff_y, diff_z, _ptr(q), _ptr(Lq))
+
+
+def Call_sspRK_mpi(numBlocks, threadsPerBlock, aStream, step, pitch, Nx, Ny, _Nz, dt, q, qo, Lq) -> None:
+    _call("Call_sspRK_mpi", _dim(numBlocks), _dim(threadsPerBlock), _stream(aStream), step, pitch, Nx, Ny, _Nz, dt,
+          _ptr(q), _ptr(qo), _ptr(Lq))
+
+
 def set_adv_scheme(weno: int = _capi.WENO_Z, p: int = 1, eps: float = 1e-6, lf: int = _capi.LF_LOCAL,
                    alpha: float = 0.0) -> None:
     """Extra Call_Adv_* parameters (defaults = the reference's compile-time choice)."""

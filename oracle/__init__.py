@@ -87,6 +87,10 @@ def lib():
         L.orc_weno7_run.argtypes = [_U, _U, _U, _D, _D, _D, _D, _P, _D, ctypes.c_int, _P, _P]
         L.orc_mb3_adv.argtypes = [ctypes.c_int, _U, _U, _U, _U, _U, _U, _D, _P, _P, _S]
         L.orc_mb3_rk.argtypes = [_U, _U, _U, _U, _U, _D, _P, _P, _P]
+        L.orc_lu3d_blk.argtypes = [_U, _U, _U, _D, _D, _D, _P, _P]
+        L.orc_adv_b5s.argtypes = [ctypes.c_int, _U, _U, _U, _D, _P, _P, _S]
+        L.orc_md3_lap.argtypes = [ctypes.c_int, _U, _U, _U, _U, _U, _U, _D, _D, _D, _P, _P]
+        L.orc_md3_rk.argtypes = [_U, _U, _U, _U, _U, _D, _P, _P, _P]
         L.orc_maxabs.restype = _D
         L.orc_maxabs.argtypes = [ctypes.c_uint64, _P]
         L.orc_rhs_periodic.argtypes = [_U, _U, _U, _D, _D, _D, _P, _P, _S]
@@ -230,6 +234,34 @@ def lu3d_o2(u, Lu, kx, ky, kz) -> None:
 def rk3d(u, uo, Lu, step, dt) -> None:
     nx, ny, nz = _shape(u)
     lib().orc_rk3d(step, nx, ny, nz, dt, _p(u), _p(uo), _p(Lu))
+
+
+# SingleGPU/Burgers3d_WENO5_SharedMem: WENO5Zreconstruction (p = 1,
+# eps = 1e-6, Burgers.h:26), local LF, the *(1/6 * (1/sum)) rounding.
+B5S = dict(weno=WENO_Z_B5S, p=1, eps=1e-6, lf=LF_LOCAL)
+
+
+def lu3d_blk(u, Lu, kx, ky, kz) -> None:
+    """Call_Lu3d of Diffusion3d_Blocking (Lu = on nodes 3..n-4, planes 3..nz-3)."""
+    lib().orc_lu3d_blk(*_shape(u), kx, ky, kz, _p(u), _p(Lu))
+
+
+def adv_b5s(axis: int, u, Lu, d, **kw) -> None:
+    """Call_Adv_{x,y,z} of Burgers3d_WENO5_SharedMem over every node (in place on Lu)."""
+    s = scheme(**(kw or B5S))
+    lib().orc_adv_b5s(axis, *_shape(u), d, _p(u), _p(Lu), ctypes.byref(s))
+
+
+def md3_lap(order: int, u, Lu, kstart, kstop, dxc, dyc, dzc, nx: int | None = None) -> None:
+    """LaplaceO2_async (order 2) / LaplaceO4_async (order 4) of the MPI heat
+    code on a (nz, ny, pitch) slab whose first nx columns are the grid."""
+    nz, ny, px = u.shape
+    lib().orc_md3_lap(order, px, px if nx is None else nx, ny, nz, kstart, kstop, dxc, dyc, dzc, _p(u), _p(Lu))
+
+
+def md3_rk(q, qo, Lq, step, dt, nx: int | None = None) -> None:
+    nz, ny, px = q.shape
+    lib().orc_md3_rk(step, px, px if nx is None else nx, ny, nz, dt, _p(q), _p(qo), _p(Lq))
 
 
 def maxabs(u) -> float:

@@ -359,6 +359,129 @@ void orc_rk3d(unsigned step, unsigned nx, unsigned ny, unsigned nz, double dt,
 }
 
 /* ------------------------------------------------------------------------ */
+/* The blocked, shared-memory and MPI drivers' variants of the operators.    */
+/* ------------------------------------------------------------------------ */
+
+/* Compute_Laplace3d_Async as Call_Lu3d of SingleGPU/Diffusion3d_Blocking
+ * launches it (kernels.cu:37-88, wrapper :187-191 with kstart = 3,
+ * kstop = nz-2): Lu = the 13-point operator with each axis summed from the
+ * low side (-u_{-2} + 16 u_{-1} - 30 u + 16 u_{+1} - u_{+2}, :69-71) on
+ * 2<i<nx-3, 2<j<ny-3, planes 3..nz-3.  Other nodes keep their Lu. */
+void orc_lu3d_blk(unsigned nx, unsigned ny, unsigned nz, double kx, double ky,
+                  double kz, const double *u, double *Lu) {
+    if (nx < 7 || ny < 7 || nz < 6) return;
+    const size_t px = nx, px2 = 2 * (size_t)nx, XY = (size_t)nx * ny, XY2 = 2 * XY;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (unsigned k = 3; k < nz - 2; k++)
+        for (unsigned j = 3; j < ny - 3; j++)
+            for (unsigned i = 3; i < nx - 3; i++) {
+                size_t o = i + px * j + XY * k;
+                double center = u[o];
+                double below2 = u[o - XY2], below = u[o - XY], above = u[o + XY], above2 = u[o + XY2];
+                Lu[o] = kx * (-u[o - 2] + 16 * u[o - 1] - 30 * center + 16 * u[o + 1] - u[o + 2]) +
+                        ky * (-u[o - px2] + 16 * u[o - px] - 30 * center + 16 * u[o + px] - u[o + px2]) +
+                        kz * (-below2 + 16 * below - 30 * center + 16 * above - above2);
+            }
+}
+
+/* Node q of a line (base, stride st, length n), 0 outside 0..n-1: the
+ * zero pad Compute_dF/dG/dH of Burgers3d_WENO5_SharedMem put beyond the
+ * array (kernels.cu:248-260, :313-325, :378-390). */
+static inline double line_node(const double *u, size_t base, size_t st, unsigned n, long q) {
+    return (q < 0 || q >= (long)n) ? 0.0 : u[base + st * (size_t)q];
+}
+
+/* WENO5Zreconstruction of the shared-memory code (kernels.cu:153-205) for
+ * the face below node l: f+ of nodes l-3..l+1 and f- of nodes l-2..l+2
+ * (its stencil s_u[si-3..si+2], :264), local LF split (:159-163). */
+static double b5s_face(const double *u, size_t base, size_t st, unsigned n, long l, const orc_scheme *s) {
+    double fp[5], fm[5], dummy;
+    for (int q = 0; q < 5; q++) {
+        orc_split(line_node(u, base, st, n, l - 3 + q), 0.0, ORC_LF_LOCAL, &fp[q], &dummy);
+        orc_split(line_node(u, base, st, n, l - 2 + q), 0.0, ORC_LF_LOCAL, &dummy, &fm[q]);
+    }
+    return orc_face(fp, fm, s);
+}
+
+/* Compute_dF (axis 0), Compute_dG (1), Compute_dH (2) of
+ * Burgers3d_WENO5_SharedMem (kernels.cu:212-402), as its driver launches
+ * them (main.cpp:94-112: every node of the array): Lu = -(F_{i+1/2} -
+ * F_{i-1/2})/dx for axis 0 (:269), Lu -= (..)/d for axes 1 and 2
+ * (:334, :399).  s->weno is ORC_WENO_Z_B5S (p = 1, eps = 1e-6 there,
+ * Burgers.h:26). */
+void orc_adv_b5s(int axis, unsigned nx, unsigned ny, unsigned nz, double d,
+                 const double *u, double *Lu, const orc_scheme *s) {
+    const size_t xy = (size_t)nx * ny;
+    const unsigned n = axis == 0 ? nx : (axis == 1 ? ny : nz);
+    const unsigned na = axis == 0 ? ny : nx, nb = axis == 2 ? ny : nz;
+    const size_t st = axis == 0 ? 1 : (axis == 1 ? nx : xy);
+#pragma omp parallel for collapse(2) schedule(static)
+    for (unsigned b = 0; b < nb; b++)
+        for (unsigned a = 0; a < na; a++) {
+            size_t base = axis == 0 ? (size_t)nx * a + xy * b
+                        : axis == 1 ? a + xy * b
+                                    : a + (size_t)nx * b;
+            for (unsigned i = 0; i < n; i++) {
+                const double flo = b5s_face(u, base, st, n, (long)i, s);
+                const double fhi = b5s_face(u, base, st, n, (long)i + 1, s);
+                const size_t o = base + st * i;
+                if (axis == 0)
+                    Lu[o] = -(fhi - flo) / d;
+                else
+                    Lu[o] -= (fhi - flo) / d;
+            }
+        }
+}
+
+/* LaplaceO2_async (order 2) and LaplaceO4_async (order 4, the one
+ * Call_Diff_ launches) of MultiGPU/Diffusion3d_Baseline (Kernels.cu:152-201,
+ * :207-260, wrapper :408-414), on a slab of nz planes with row pitch px:
+ *   O2: Lu = dx_ (u_w - 2u + u_e) + dy_ (u_s - 2u + u_n) + dz_ (u_b - 2u + u_t)
+ *   O4: Lu = dx_/12 (-u_ww + 16u_w + 16u_e - u_ee - u_ss + 16u_s - 90u
+ *                    + 16u_n - u_nn - u_bb + 16u_b + 16u_t - u_tt)
+ * (dy_, dz_ unused by O4) on 2<i<nx-3, 2<j<ny-3, planes kstart..kstop-1. */
+void orc_md3_lap(int order, unsigned px, unsigned nx, unsigned ny, unsigned nz,
+                 unsigned kstart, unsigned kstop, double dxc, double dyc,
+                 double dzc, const double *u, double *Lu) {
+    const size_t XY = (size_t)px * ny, XY2 = 2 * XY, px2 = 2 * (size_t)px;
+    if (nx < 7 || ny < 7 || kstop > nz) return;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (unsigned k = kstart; k < kstop; k++)
+        for (unsigned j = 3; j < ny - 3; j++)
+            for (unsigned i = 3; i < nx - 3; i++) {
+                const size_t o = i + (size_t)px * j + XY * k;
+                const double center = u[o];
+                if (order == 2) {
+                    const double below = u[o - XY], above = u[o + XY];
+                    Lu[o] = dxc * (u[o - 1] - 2 * center + u[o + 1]) +
+                            dyc * (u[o - px] - 2 * center + u[o + px]) +
+                            dzc * (below - 2 * center + above);
+                } else {
+                    const double below2 = u[o - XY2], below = u[o - XY], above = u[o + XY], above2 = u[o + XY2];
+                    Lu[o] = dxc / 12 * (
+                        - u[o - 2] + 16 * u[o - 1] + 16 * u[o + 1] - u[o + 2]
+                        - u[o - px2] + 16 * u[o - px] - 90 * center + 16 * u[o + px] - u[o + px2]
+                        - below2 + 16 * below + 16 * above - above2);
+                }
+            }
+}
+
+/* Compute_RK of MultiGPU/Diffusion3d_Baseline (Kernels.cu:266-297, Call_sspRK
+ * :401-406): SSP-RK3 stage on 2<i<nx-3, 2<j<ny-3 and every plane k < nz
+ * (the slab's ghost planes advance too). */
+void orc_md3_rk(unsigned step, unsigned px, unsigned nx, unsigned ny, unsigned nz,
+                double dt, double *u, const double *uo, const double *Lu) {
+    if (nx < 7 || ny < 7) return;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (unsigned k = 0; k < nz; k++)
+        for (unsigned j = 3; j < ny - 3; j++)
+            for (unsigned i = 3; i < nx - 3; i++) {
+                const size_t o = i + (size_t)px * j + (size_t)px * ny * k;
+                u[o] = rk_node(step, dt, u[o], uo[o], Lu[o]);
+            }
+}
+
+/* ------------------------------------------------------------------------ */
 /* MultiGPU/Burgers3d_Baseline: one rank's z slab (pitched)                 */
 /* ------------------------------------------------------------------------ */
 

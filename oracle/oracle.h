@@ -112,6 +112,17 @@ int    orc_burgers_run(unsigned nx, unsigned ny, unsigned nz, double dx,
                        double tend, int nsteps, const orc_scheme *s,
                        double *u);
 
+/* ---- blocked / shared-memory / MPI driver variants ---- */
+void orc_lu3d_blk(unsigned nx, unsigned ny, unsigned nz, double kx, double ky,
+                  double kz, const double *u, double *Lu);
+void orc_adv_b5s(int axis, unsigned nx, unsigned ny, unsigned nz, double d,
+                 const double *u, double *Lu, const orc_scheme *s);
+void orc_md3_lap(int order, unsigned px, unsigned nx, unsigned ny, unsigned nz,
+                 unsigned kstart, unsigned kstop, double dxc, double dyc,
+                 double dzc, const double *u, double *Lu);
+void orc_md3_rk(unsigned step, unsigned px, unsigned nx, unsigned ny, unsigned nz,
+                double dt, double *u, const double *uo, const double *Lu);
+
 /* ---- heat ---- */
 void orc_heat7_fe(unsigned nx, unsigned ny, unsigned nz, double kx, double ky,
                   double kz, double dt, int nsteps, double *u);

@@ -1,0 +1,227 @@
+"""The five workloads of the benchmark set at their stated sizes, each checked
+against the CPU oracle or by properties that hold at any size.
+
+- WENO5-JS (eps 1e-6, global LF) 256^3 to t = 0.5 from the seeded Fourier
+  field (seed 2): bit-exact against the oracle over the whole run.  The GPU
+  run keeps its state at six step boundaries; the oracle advances each
+  segment from the GPU's state at its start and must land on the GPU's state
+  at its end, step count and t included.  Since the oracle is deterministic,
+  six exact segments starting from the same initial field are the oracle's
+  whole run, cut into six tests of a few tens of seconds each.
+- WENO5-Z p=2 eps=1e-40 512^3 (seed 3): one FP64 step bit-exact against the
+  oracle; the run to t = 0.5 keeps sum(u) (periodic conservation), stays
+  finite and near the initial range, and is exchanged bit for bit by an x<->y
+  transpose of the initial field (10 steps); the FP32-storage run's L1 / Linf
+  error against the FP64 run is bounded before and after the shocks.
+- 64 uneven problems, edges from {32,48,64,96,128} (seed 4, fields 4+i), 200
+  hipGraph steps: the batched persistent launches equal the problems advanced
+  one by one for all 64, and one problem of each size equals the oracle.
+- Adjoint at 128^3 over 400 steps with sqrt(N) checkpoints: J and dJ/du0
+  against the oracle's reverse-mode adjoint (which keeps every stage input).
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+from multigpu_advectiondiffusion_amd import solvers, studies
+
+pytestmark = pytest.mark.gpu
+
+NSEG = 6
+
+
+# ---------------------------------------------------------------------------
+# WENO5-JS, 256^3, to t = 0.5
+# ---------------------------------------------------------------------------
+@pytest.fixture(scope="module")
+def js256():
+    n, tend = 256, 0.5
+    u0 = solvers.ic_fourier(n, seed=2)
+    s = solvers.PeriodicBurgers(n, scheme="JS", cfl=0.4)
+    s.set_initial(u0, tend=tend)
+    total = s.run(batch=64).steps
+    seg = math.ceil(total / NSEG)
+    s.set_initial(u0, tend=tend)
+    states = [(u0, 0.0, 0)]
+    for _ in range(NSEG):
+        s.steps(seg)
+        st = s.state()
+        states.append((s.solution(), st.t, st.steps))
+    assert states[-1][1] == tend and states[-1][2] == total
+    return dict(n=n, tend=tend, seg=seg, states=states)
+
+
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("seg", range(NSEG))
+def test_js256_to_t05_segment_bitexact(hipdev, js256, seg):
+    n, tend = js256["n"], js256["tend"]
+    u, t, k = js256["states"][seg]
+    g, tg, kg = js256["states"][seg + 1]
+    ref = u.copy()
+    steps, tr = O.burgers_run(ref, 2.0 / n, 0.4, t, tend, nsteps=js256["seg"], **O.JS_GLOBAL)
+    assert steps == kg - k and tr == tg
+    assert np.array_equal(g, ref), float(np.max(np.abs(g - ref)))
+
+
+# ---------------------------------------------------------------------------
+# WENO5-Z p=2, 512^3
+# ---------------------------------------------------------------------------
+N512 = 512
+
+
+@pytest.fixture(scope="module")
+def u512():
+    return solvers.ic_fourier(N512, seed=3)
+
+
+@pytest.mark.timeout(300)
+def test_z2_512_one_step_bitexact(hipdev, u512):
+    s = solvers.PeriodicBurgers(N512, scheme="Z2")
+    s.set_initial(u512, tend=0.5)
+    s.steps(1)
+    g = s.solution()
+    st = s.state()
+    del s
+    ref = u512.copy()
+    dt = O.burgers_step(ref, 2.0 / N512, 0.4, 0.0, 0.5, **O.Z2_GLOBAL)
+    assert st.dt == dt and st.t == dt
+    assert np.array_equal(g, ref), float(np.max(np.abs(g - ref)))
+
+
+@pytest.fixture(scope="module")
+def z2_512_f64(u512):
+    s = solvers.PeriodicBurgers(N512, scheme="Z2")
+    s.set_initial(u512, tend=0.5)
+    st = s.run(batch=64)
+    return s.solution(), st
+
+
+@pytest.mark.timeout(300)
+def test_z2_512_run_to_t05_properties(hipdev, u512, z2_512_f64):
+    u, st = z2_512_f64
+    assert st.t == 0.5 and st.steps > 300
+    assert np.all(np.isfinite(u))
+    # the entropy solution stays in the initial range; WENO's oscillations
+    # at shocks are a small fraction of the jumps
+    lo, hi = float(u512.min()), float(u512.max())
+    assert u.min() >= lo - 0.05 * (hi - lo) and u.max() <= hi + 0.05 * (hi - lo), (u.min(), u.max())
+    # periodic flux differences telescope: the mean is conserved (the
+    # per-node roundings of 3*steps stages add up to far below 1e-12)
+    m0, m1 = math.fsum(u512.ravel()) / u512.size, math.fsum(u.ravel()) / u.size
+    assert abs(m1 - m0) <= 1e-12 * abs(m0), (m0, m1)
+
+
+def _swap_xy(u):
+    return np.ascontiguousarray(np.swapaxes(u, 1, 2))
+
+
+@pytest.mark.timeout(300)
+def test_z2_512_xy_exchange_bitwise(hipdev, u512):
+    """Exchanging x and y exchanges dF and dG, and (-a) - b = (-b) - a in
+    IEEE arithmetic: the transposed start gives the transposed run."""
+    a = solvers.PeriodicBurgers(N512, scheme="Z2")
+    a.set_initial(u512, tend=0.5)
+    a.steps(10)
+    ga = _swap_xy(a.solution())
+    del a
+    b = solvers.PeriodicBurgers(N512, scheme="Z2")
+    b.set_initial(_swap_xy(u512), tend=0.5)
+    b.steps(10)
+    assert np.array_equal(ga, b.solution())
+
+
+def _shock_time(u0, h):
+    """1 / max(-(u_x + u_y + u_z)) of the initial field: along the
+    characteristics of u_t + u (u_x + u_y + u_z) = 0 the derivative obeys
+    d(Du)/dt = -(Du)^2, so the first shock forms then."""
+    D = sum((np.roll(u0, -1, axis=a) - np.roll(u0, 1, axis=a)) / (2 * h) for a in range(3))
+    return 1.0 / float(np.max(-D))
+
+
+@pytest.mark.timeout(300)
+def test_z2_512_fp32_storage_error(hipdev, u512, z2_512_f64):
+    """FP32 storage, FP64 arithmetic.  Before the first shock (half its
+    formation time, about 0.015 here) the difference is storage rounding
+    carried along smooth characteristics: a few float ulps of max|u| per
+    stage.  At t = 0.5 it is dominated by shocks moved by a fraction of a
+    cell, which only a thin set of nodes sees (small L1, Linf below the
+    largest jump)."""
+    u64, st64 = z2_512_f64
+    t_pre = 0.5 * _shock_time(u512, 2.0 / N512)
+    assert 0.0 < t_pre < 0.1
+    pre = {}
+    for prec in ("f64", "f32"):
+        s = solvers.PeriodicBurgers(N512, scheme="Z2", precision=prec)
+        s.set_initial(u512, tend=t_pre)
+        st = s.run(batch=64)
+        assert st.t == t_pre
+        pre[prec] = s.solution().astype(np.float64)
+        del s
+    d = np.abs(pre["f32"] - pre["f64"])
+    assert np.max(d) <= 128 * np.finfo(np.float32).eps * np.max(np.abs(pre["f64"])), float(np.max(d))
+    s = solvers.PeriodicBurgers(N512, scheme="Z2", precision="f32")
+    s.set_initial(u512, tend=0.5)
+    st = s.run(batch=64)
+    u32 = s.solution().astype(np.float64)
+    del s
+    assert st.t == 0.5 and abs(st.steps - st64.steps) <= 2
+    d = np.abs(u32 - u64)
+    l1, linf = float(np.mean(d)), float(np.max(d))
+    assert l1 <= 3e-4, l1
+    assert linf <= 0.25, linf
+    assert np.mean(d > 1e-3) <= 0.01  # the shocks are a thin set of nodes
+
+
+# ---------------------------------------------------------------------------
+# 64 uneven problems, 200 steps
+# ---------------------------------------------------------------------------
+@pytest.mark.timeout(600)
+def test_batched_64_problems_200_steps(hipdev):
+    sizes = studies.batch_sizes()
+    assert len(sizes) == 64 and set(sizes) <= {32, 48, 64, 96, 128}
+    u0s = [solvers.ic_fourier(n, seed=4 + i) for i, n in enumerate(sizes)]
+    b = solvers.BurgersBatch(sizes, scheme="Z2")
+    b.set_initial(u0s, tend=float("inf"), kchunk=0)
+    b.steps(200, graph=True)
+    got, sts = b.solutions(), b.states()
+    b.set_initial(u0s, tend=float("inf"), kchunk=b.batch.kchunk)
+    b.steps_serial(200, graph=True)
+    alone, sts1 = b.solutions(), b.states()
+    for i in range(64):
+        assert sts[i].steps == 200 and sts[i].t == sts1[i].t
+        assert np.array_equal(got[i], alone[i]), (i, sizes[i])
+    for n in sorted(set(sizes)):
+        i = sizes.index(n)
+        ref = u0s[i].copy()
+        steps, t = O.burgers_run(ref, 2.0 / n, 0.4, 0.0, float("inf"), nsteps=200, **O.Z2_GLOBAL)
+        assert steps == 200 and t == sts[i].t
+        assert np.array_equal(got[i], ref), (i, n)
+
+
+# ---------------------------------------------------------------------------
+# Adjoint, 128^3, 400 steps, sqrt(N) checkpoints
+# ---------------------------------------------------------------------------
+@pytest.mark.timeout(900)
+def test_adjoint_128_400_steps_vs_oracle(hipdev):
+    n, nsteps = 128, 400
+    u0 = solvers.ic_fourier(n, seed=5)
+    noise = np.random.default_rng(5).uniform(-1.0, 1.0, size=u0.shape)
+    s = solvers.PeriodicBurgers(n, scheme="Z2")
+    s.set_initial(u0, tend=float("inf"))
+    s.steps(nsteps)
+    T = s.state().t
+    s.set_initial(u0 + 1e-2 * noise, tend=T)
+    s.run(batch=64)
+    target = s.solution()
+    adj = solvers.BurgersAdjoint(n, scheme="Z2", max_steps=nsteps + 8)
+    assert adj.every == math.ceil(math.sqrt(nsteps + 8))
+    J, g, steps = adj(u0, target, T)
+    assert steps == nsteps
+    Jr, gr, steps_r = O.burgers_adjoint(u0, target, 2.0 / n, 0.4, 0.0, T, weno=O.WENO_Z, p=2, eps=1e-40)
+    assert steps_r == nsteps
+    assert abs(J - Jr) <= 1e-12 * abs(Jr), (J, Jr)
+    err = float(np.max(np.abs(g - gr)))
+    assert err <= 1e-9 * float(np.max(np.abs(gr))), err
+    assert float(np.linalg.norm(g - gr)) <= 1e-9 * float(np.linalg.norm(gr))
