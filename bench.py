@@ -90,31 +90,54 @@ def run_to_half(scheme: str, n: int = 256, seed: int = 3) -> dict:
     return {"steps": st.steps, "t": st.t, "wall_s": round(wall, 4), "gpts": round(n ** 3 * st.steps / wall / 1e9, 3)}
 
 
+def _cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.lower().startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(n: int, scheme: str, seed: int):
-    """The CPU oracle (OpenMP restatement of the reference routines) timed on a
-    bounded sample of the same workload: whole SSP-RK3 steps of the n^3 grid,
-    as many as fit in about 10 s (at most 20)."""
+    """The CPU oracle (OpenMP restatement of the reference routines), built
+    -O3 -march=native for this host, timed on a bounded sample of the same
+    workload: whole SSP-RK3 steps of the n^3 grid, as many as fit in about
+    10 s (at most 20).  Threads are placed one per core, close together."""
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
     import numpy as np
 
     import oracle as O
     from multigpu_advectiondiffusion_amd import solvers
 
     kw = {"Z2": O.Z2_GLOBAL, "JS": O.JS_GLOBAL}[scheme]
+    # the native build computes the same bits as the tested one
+    a = solvers.ic_fourier(32, seed=seed)
+    b = a.copy()
+    O.burgers_step(a, 2.0 / 32, 0.4, 0.0, 10.0, **kw)
+    O.burgers_step(b, 2.0 / 32, 0.4, 0.0, 10.0, native=True, **kw)
+    assert np.array_equal(a, b), "native oracle build differs from the tested one"
     u = solvers.ic_fourier(n, seed=seed)
-    O.burgers_step(u[:8, :8, :8].copy(), 2.0 / 8, 0.4, 0.0, 10.0, **kw)  # load the library
     steps, t = 0, 0.0
     t0 = time.perf_counter()
     while steps < 20 and (t < 10.0 or steps == 0):
-        O.burgers_step(u, 2.0 / n, 0.4, 0.0, 10.0, **kw)
+        O.burgers_step(u, 2.0 / n, 0.4, 0.0, 10.0, native=True, **kw)
         steps += 1
         t = time.perf_counter() - t0
     assert np.isfinite(u).all()
     return {
         "value": round(n ** 3 * steps / t / 1e9, 5),
         "unit": "Gpts/s",
-        "cores": O.num_threads(),
+        "cores": O.num_threads(native=True),
         "kind": "port",
         "sample": f"{steps} SSP-RK3 steps of the {n}^3 {scheme} periodic grid in {t:.2f} s",
+        "build": "gcc -O3 -march=native -fopenmp -ffp-contract=off (oracle/Makefile native)",
+        "cpu_model": _cpu_model(),
+        "host_cpus": os.cpu_count(),
+        "affinity_cpus": len(os.sched_getaffinity(0)),
+        "omp": {k: os.environ.get(k) for k in ("OMP_NUM_THREADS", "OMP_PROC_BIND", "OMP_PLACES")},
     }
 
 
@@ -228,13 +251,18 @@ def main() -> int:
                                              "results_identical")}
         print("batched study done", file=sys.stderr, flush=True)
         a = studies.adjoint(n=128)
+        # per-direction |FD - AD| / |AD| over the 16 directions, central
+        # differences at h = 1e-6 and their Richardson extrapolation
+        pick = ("max_rel_err", "median_rel_err")
         out["adjoint"] = {"cost_ratio": a["cost_ratio"], "steps": a["cost_steps"], "grid": 128,
                           "checkpoint_every": a["sqrtN"]["checkpoint_every"],
                           "cost_ratio_store_all": a["store_all"]["cost_ratio"],
-                          "fd_max_err_over_grad_norm_h1e-6": max(a["T0.2"]["h1e-06"]["max_err_over_grad_norm"],
-                                                                 a["T0.5"]["h1e-06"]["max_err_over_grad_norm"]),
-                          "fd_median_rel_err_h1e-6": max(a["T0.2"]["h1e-06"]["median_rel_err"],
-                                                         a["T0.5"]["h1e-06"]["median_rel_err"])}
+                          "fd_central_h1e-6": {T: {k: a[T]["central"]["h1e-06"][k] for k in pick}
+                                               for T in ("T0.2", "T0.5")},
+                          "fd_richardson_h1e-6": {T: {k: a[T]["richardson"]["h1e-06"][k] for k in pick}
+                                                  for T in ("T0.2", "T0.5")},
+                          "fd_central_max_rel_err_by_h": {T: {h: v["max_rel_err"] for h, v in a[T]["central"].items()}
+                                                          for T in ("T0.2", "T0.5")}}
         print("adjoint study done", file=sys.stderr, flush=True)
         r = studies.reference_cases()
         out["reference_cases"] = [{k: c[k] for k in ("grid", "steps", "fused_gpts", "published_gpts",

@@ -58,58 +58,76 @@ _P = ctypes.POINTER(ctypes.c_double)
 _S = ctypes.POINTER(Scheme)
 
 
+_NATIVE_PATH = os.path.join(_HERE, "_native", "liboracle_native.so")
+_native = None
+
+
+def native_lib():
+    """The same oracle built -O3 -march=native for this machine (built here on
+    first use; for timing the CPU baseline, never shipped)."""
+    global _native
+    if _native is None:
+        subprocess.run(["make", "-s", "-C", _HERE, "native"], check=True)
+        _native = _setup(ctypes.CDLL(_NATIVE_PATH))
+    return _native
+
+
 def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(_LIB_PATH):
             build()
-        L = ctypes.CDLL(_LIB_PATH)
-        L.orc_flux.restype = _D
-        L.orc_flux.argtypes = [_D]
-        L.orc_split.argtypes = [_D, _D, ctypes.c_int, _P, _P]
-        L.orc_face.restype = _D
-        L.orc_face.argtypes = [_P, _P, _S]
-        L.orc_weno_plus.restype = _D
-        L.orc_weno_plus.argtypes = [_P, _S]
-        L.orc_weno_minus.restype = _D
-        L.orc_weno_minus.argtypes = [_P, _S]
-        for name in ("orc_adv_x", "orc_adv_y", "orc_adv_z"):
-            getattr(L, name).argtypes = [_U, _U, _U, _D, _P, _P, _S]
-        L.orc_diff13_b5.argtypes = [_U, _U, _U, _D, _D, _D, _P, _P]
-        L.orc_ssprk_b5.argtypes = [_U, _U, _U, _U, _D, _P, _P, _P]
-        L.orc_lu3d_13.argtypes = [_U, _U, _U, _D, _D, _D, _P, _P]
-        L.orc_lu3d_o2.argtypes = [_U, _U, _U, _D, _D, _D, _P, _P]
-        L.orc_rk3d.argtypes = [_U, _U, _U, _U, _D, _P, _P, _P]
-        L.orc_weno7_face.argtypes = [_P, _P, _P, _P]
-        L.orc_weno7_rhs.argtypes = [_U, _U, _U, _D, _D, _D, _P, _P]
-        L.orc_lsrk_stage.argtypes = [ctypes.c_uint64, ctypes.c_int, _D, _P, _P, _P]
-        L.orc_weno7_run.restype = ctypes.c_int
-        L.orc_weno7_run.argtypes = [_U, _U, _U, _D, _D, _D, _D, _P, _D, ctypes.c_int, _P, _P]
-        L.orc_mb3_adv.argtypes = [ctypes.c_int, _U, _U, _U, _U, _U, _U, _D, _P, _P, _S]
-        L.orc_mb3_rk.argtypes = [_U, _U, _U, _U, _U, _D, _P, _P, _P]
-        L.orc_lu3d_blk.argtypes = [_U, _U, _U, _D, _D, _D, _P, _P]
-        L.orc_adv_b5s.argtypes = [ctypes.c_int, _U, _U, _U, _D, _P, _P, _S]
-        L.orc_md3_lap.argtypes = [ctypes.c_int, _U, _U, _U, _U, _U, _U, _D, _D, _D, _P, _P]
-        L.orc_md3_rk.argtypes = [_U, _U, _U, _U, _U, _D, _P, _P, _P]
-        L.orc_maxabs.restype = _D
-        L.orc_maxabs.argtypes = [ctypes.c_uint64, _P]
-        L.orc_rhs_periodic.argtypes = [_U, _U, _U, _D, _D, _D, _P, _P, _S]
-        L.orc_rk_stage_all.argtypes = [ctypes.c_uint64, _U, _D, _P, _P, _P]
-        L.orc_burgers_step.restype = _D
-        L.orc_burgers_step.argtypes = [_U, _U, _U, _D, _D, _D, _D, _D, _D, _S, _P, _P]
-        L.orc_burgers_run.restype = ctypes.c_int
-        L.orc_burgers_run.argtypes = [_U, _U, _U, _D, _D, _D, _D, _P, _D, ctypes.c_int, _S, _P]
-        L.orc_heat7_fe.argtypes = [_U, _U, _U, _D, _D, _D, _D, ctypes.c_int, _P]
-        L.orc_rhs_1d_periodic.argtypes = [_U, _D, _P, _P, _S]
-        L.orc_side_grad.restype = _D
-        L.orc_side_grad.argtypes = [_P, ctypes.c_int, _S, _P]
-        L.orc_rhs_vjp.argtypes = [_U, _U, _U, _D, _D, _D, _P, _P, _P, _S, _P, _P]
-        L.orc_burgers_adjoint.restype = _D
-        L.orc_burgers_adjoint.argtypes = [_U, _U, _U, _D, _D, _D, _D, _D, _D, _S, _P, _P, _P,
-                                          ctypes.POINTER(ctypes.c_int)]
-        L.orc_num_threads.restype = ctypes.c_int
-        _lib = L
+        _lib = _setup(ctypes.CDLL(_LIB_PATH))
     return _lib
+
+
+def _setup(L):
+    """Declares the argument and result types of every oracle function."""
+    L.orc_flux.restype = _D
+    L.orc_flux.argtypes = [_D]
+    L.orc_split.argtypes = [_D, _D, ctypes.c_int, _P, _P]
+    L.orc_face.restype = _D
+    L.orc_face.argtypes = [_P, _P, _S]
+    L.orc_weno_plus.restype = _D
+    L.orc_weno_plus.argtypes = [_P, _S]
+    L.orc_weno_minus.restype = _D
+    L.orc_weno_minus.argtypes = [_P, _S]
+    for name in ("orc_adv_x", "orc_adv_y", "orc_adv_z"):
+        getattr(L, name).argtypes = [_U, _U, _U, _D, _P, _P, _S]
+    L.orc_diff13_b5.argtypes = [_U, _U, _U, _D, _D, _D, _P, _P]
+    L.orc_ssprk_b5.argtypes = [_U, _U, _U, _U, _D, _P, _P, _P]
+    L.orc_lu3d_13.argtypes = [_U, _U, _U, _D, _D, _D, _P, _P]
+    L.orc_lu3d_o2.argtypes = [_U, _U, _U, _D, _D, _D, _P, _P]
+    L.orc_rk3d.argtypes = [_U, _U, _U, _U, _D, _P, _P, _P]
+    L.orc_weno7_face.argtypes = [_P, _P, _P, _P]
+    L.orc_weno7_rhs.argtypes = [_U, _U, _U, _D, _D, _D, _P, _P]
+    L.orc_lsrk_stage.argtypes = [ctypes.c_uint64, ctypes.c_int, _D, _P, _P, _P]
+    L.orc_weno7_run.restype = ctypes.c_int
+    L.orc_weno7_run.argtypes = [_U, _U, _U, _D, _D, _D, _D, _P, _D, ctypes.c_int, _P, _P]
+    L.orc_mb3_adv.argtypes = [ctypes.c_int, _U, _U, _U, _U, _U, _U, _D, _P, _P, _S]
+    L.orc_mb3_rk.argtypes = [_U, _U, _U, _U, _U, _D, _P, _P, _P]
+    L.orc_lu3d_blk.argtypes = [_U, _U, _U, _D, _D, _D, _P, _P]
+    L.orc_adv_b5s.argtypes = [ctypes.c_int, _U, _U, _U, _D, _P, _P, _S]
+    L.orc_md3_lap.argtypes = [ctypes.c_int, _U, _U, _U, _U, _U, _U, _D, _D, _D, _P, _P]
+    L.orc_md3_rk.argtypes = [_U, _U, _U, _U, _U, _D, _P, _P, _P]
+    L.orc_maxabs.restype = _D
+    L.orc_maxabs.argtypes = [ctypes.c_uint64, _P]
+    L.orc_rhs_periodic.argtypes = [_U, _U, _U, _D, _D, _D, _P, _P, _S]
+    L.orc_rk_stage_all.argtypes = [ctypes.c_uint64, _U, _D, _P, _P, _P]
+    L.orc_burgers_step.restype = _D
+    L.orc_burgers_step.argtypes = [_U, _U, _U, _D, _D, _D, _D, _D, _D, _S, _P, _P]
+    L.orc_burgers_run.restype = ctypes.c_int
+    L.orc_burgers_run.argtypes = [_U, _U, _U, _D, _D, _D, _D, _P, _D, ctypes.c_int, _S, _P]
+    L.orc_heat7_fe.argtypes = [_U, _U, _U, _D, _D, _D, _D, ctypes.c_int, _P]
+    L.orc_rhs_1d_periodic.argtypes = [_U, _D, _P, _P, _S]
+    L.orc_side_grad.restype = _D
+    L.orc_side_grad.argtypes = [_P, ctypes.c_int, _S, _P]
+    L.orc_rhs_vjp.argtypes = [_U, _U, _U, _D, _D, _D, _P, _P, _P, _S, _P, _P]
+    L.orc_burgers_adjoint.restype = _D
+    L.orc_burgers_adjoint.argtypes = [_U, _U, _U, _D, _D, _D, _D, _D, _D, _S, _P, _P, _P,
+                                      ctypes.POINTER(ctypes.c_int)]
+    L.orc_num_threads.restype = ctypes.c_int
+    return L
 
 
 def _p(a: np.ndarray):
@@ -123,8 +141,8 @@ def _shape(u: np.ndarray):
     return nx, ny, nz
 
 
-def num_threads() -> int:
-    return lib().orc_num_threads()
+def num_threads(native: bool = False) -> int:
+    return (native_lib() if native else lib()).orc_num_threads()
 
 
 def flux(u: float) -> float:
@@ -280,11 +298,13 @@ def rk_stage_all(u, uo, Lu, stage, dt) -> None:
     lib().orc_rk_stage_all(u.size, stage, dt, _p(u), _p(uo), _p(Lu))
 
 
-def burgers_step(u, d, cfl, t, tend, **kw) -> float:
+def burgers_step(u, d, cfl, t, tend, native: bool = False, **kw) -> float:
+    """One SSP-RK3 step in place; returns dt.  native: through native_lib()."""
     dx, dy, dz = (d, d, d) if np.isscalar(d) else d
     work = np.empty(2 * u.size)
     s = scheme(**kw)
-    return lib().orc_burgers_step(*_shape(u), dx, dy, dz, cfl, t, tend, ctypes.byref(s), _p(u), _p(work))
+    L = native_lib() if native else lib()
+    return L.orc_burgers_step(*_shape(u), dx, dy, dz, cfl, t, tend, ctypes.byref(s), _p(u), _p(work))
 
 
 def burgers_run(u, d, cfl, t, tend, nsteps=-1, **kw):

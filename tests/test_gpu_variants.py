@@ -119,9 +119,10 @@ def test_blocking_lu3d(hipdev, shape, pitch):
 def test_blocking_driver_freezes_the_second_ring(hipdev):
     """Diffusion3d_Blocking/main.cpp:101-122: Lu zeroed once, then per step
     uo <- u and per stage Call_Lu3d + Call_RK3d.  Call_RK3d updates nodes
-    2..n-3 but Call_Lu3d writes only 3..n-4, so on the ring of index 2 (and
-    n-3) the stages see Lu = 0 and return their input up to the rounding of
-    0.75 u + 0.25 u and (u + 2u)/3."""
+    2..n-3 but Call_Lu3d writes only x and y nodes 3..n-4 and planes
+    3..nz-3, so on the rest of the updated region (x or y index 2 or n-3,
+    plane 2) the stages see Lu = 0 and return their input up to the rounding
+    of 0.75 u + 0.25 u and (u + 2u)/3."""
     nx, ny, nz, pitch = 30, 26, 22, 32
     h = 1.0 / 29
     kx = ky = kz = 1.0 / (12 * h * h)
@@ -142,8 +143,8 @@ def test_blocking_driver_freezes_the_second_ring(hipdev):
             O.rk3d(u, uo, Lu, step, dt)
     check(du, u, pu, nx)
     ring = np.zeros(u.shape, bool)
-    ring[2:-2, 2:-2, 2:-2] = True
-    ring[3:-3, 3:-3, 3:-3] = False
+    ring[2:-2, 2:-2, 2:-2] = True    # Call_RK3d's region
+    ring[3:-2, 3:-3, 3:-3] = False   # Call_Lu3d's (planes 3..nz-3)
     assert np.max(np.abs(u[ring] - u0[ring]) / np.abs(u0[ring])) <= 1e-14
     assert np.max(np.abs(u[3:-3, 3:-3, 3:-3] - u0[3:-3, 3:-3, 3:-3])) > 1e-3
 
