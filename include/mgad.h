@@ -38,7 +38,9 @@ typedef enum {
     MGAD_ERR_INVALID_ARG = 1,   /* bad size, null pointer, bad stage number */
     MGAD_ERR_UNSUPPORTED = 2,   /* grid shape not supported by the kernel set */
     MGAD_ERR_HIP = 3,           /* a HIP runtime call or a launch failed */
-    MGAD_ERR_NONFINITE = 4      /* max|u| read back as NaN or Inf */
+    MGAD_ERR_NONFINITE = 4,     /* max|u| read back as NaN or Inf */
+    MGAD_ERR_SCRATCH_FULL = 5   /* input needs more than the library's static
+                                   device scratch (mgad_batch_init_static) */
 } mgad_status;
 
 /* Returns the first error recorded since the previous call and clears it. */
@@ -422,6 +424,30 @@ int mgad_batch_steps_serial(hipStream_t stream, const mgad_batch *batch,
                             const mgad_problem *probs, int nsteps,
                             int use_graph);
 
+/* Size of the library's static device scratch that holds the work list of
+ * mgad_batch_init_static (descriptors, items, shares): 1 MiB per GPU, held
+ * from library load until the process ends. */
+#define MGAD_BATCH_SCRATCH_BYTES (1u << 20)
+
+/* No counterpart in the reference.  mgad_batch_init for callers that cannot
+ * allocate device memory (code under HIP graph capture set-up, serving loops,
+ * harnesses that forbid hipMalloc): the same work list, written into the
+ * library's static scratch of the current device instead of a caller
+ * workspace, and no device allocation (no hipMalloc, hipFree, events or
+ * graphs).  kchunk must name the list (MGAD_BATCH_SPLIT, MGAD_BATCH_SPLIT_WIDE
+ * or planes per item > 0): 0 (autotune) is rejected, since timing needs HIP
+ * events.  A work list above MGAD_BATCH_SCRATCH_BYTES returns
+ * MGAD_ERR_SCRATCH_FULL.  out->workspace points into the scratch, so
+ * mgad_batch_steps(use_graph = 0) then runs without allocating either.  One
+ * scratch per GPU: the next mgad_batch_init_static on the same device
+ * replaces the work list, so a batch from it is valid until then, and calls
+ * that overlap on one device (other host threads, or steps of an earlier
+ * static batch still queued on another stream) race.  Synchronises the
+ * stream. */
+int mgad_batch_init_static(hipStream_t stream, int precision,
+                           const mgad_scheme *scheme, int nprob,
+                           const mgad_problem *probs, int kchunk, double t0,
+                           double tend, double cfl, mgad_batch *out);
 
 /* ------------------------------------------------------------------------ */
 /* z-slab decomposition of the periodic Burgers run (FP64)                   */

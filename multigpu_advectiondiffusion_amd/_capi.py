@@ -12,6 +12,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libmgad.so")
 
 MGAD_OK, MGAD_ERR_INVALID_ARG, MGAD_ERR_UNSUPPORTED, MGAD_ERR_HIP, MGAD_ERR_NONFINITE = 0, 1, 2, 3, 4
+MGAD_ERR_SCRATCH_FULL = 5
+BATCH_SCRATCH_BYTES = 1 << 20  # MGAD_BATCH_SCRATCH_BYTES
 WENO_Z, WENO_JS = 0, 1
 LF_LOCAL, LF_GLOBAL = 0, 1
 CTRL_BYTES = 256
@@ -31,6 +33,7 @@ EXPORTS = (
     "mgad_burgers_init", "mgad_burgers_stage", "mgad_burgers_steps", "mgad_burgers_read_ctrl",
     "mgad_burgers_init_p", "mgad_burgers_steps_p", "mgad_burgers_autotune", "mgad_burgers_set_launch",
     "mgad_batch_workspace_bytes", "mgad_batch_init", "mgad_batch_steps", "mgad_batch_steps_serial",
+    "mgad_batch_init_static",
     "mgad_adjoint_workspace_bytes", "mgad_burgers_adjoint",
     "mgad_burgers_ref_steps", "mgad_heat7_fe_steps", "mgad_heat13_rk3_steps", "mgad_check_division", "mgad_ic_fourier", "mgad_ic_gauss_noise",
     "mgad_save_binary3d", "mgad_load_binary3d", "mgad_init3d_burgers", "mgad_init3d_heat", "mgad_heat_error_norms",
@@ -163,6 +166,8 @@ def lib():
         L.mgad_batch_workspace_bytes.restype = ctypes.c_size_t
         L.mgad_batch_init.argtypes = [VP, I, ctypes.POINTER(Scheme), I, PP, VP, ctypes.c_size_t, I, D, D, D, BP]
         L.mgad_batch_init.restype = I
+        L.mgad_batch_init_static.argtypes = [VP, I, ctypes.POINTER(Scheme), I, PP, I, D, D, D, BP]
+        L.mgad_batch_init_static.restype = I
         L.mgad_batch_steps.argtypes = [VP, BP, I, I]
         L.mgad_batch_steps.restype = I
         L.mgad_batch_steps_serial.argtypes = [VP, BP, PP, I, I]

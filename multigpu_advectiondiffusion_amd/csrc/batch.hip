@@ -188,6 +188,24 @@ void cut_shares(const std::vector<unsigned> &planes, long nshares, uint2 *shares
     shares[nshares] = make_uint2(static_cast<unsigned>(planes.size()), 0u);
 }
 
+// Persistent blocks of a list's stage launch: the resident blocks, or fewer
+// when the list has less work (tile planes of a split list, items of a
+// chunked one).
+long list_grid(int nprob, const mgad_problem *p, int kchunk) {
+    const int tw = tile_w(kchunk);
+    long work = 0;
+    if (is_split(kchunk))
+        for (int i = 0; i < nprob; i++) work += col_tiles(p[i], tw) * p[i].nz;
+    else
+        work = count_items(nprob, p, kchunk);
+    return std::max(1l, std::min(work, static_cast<long>(resident_blocks(tw))));
+}
+
+// Workspace bytes of one list (the shares of a split list are one per block).
+size_t list_bytes(int nprob, const mgad_problem *p, int kchunk) {
+    return ws_bytes(nprob, count_items(nprob, p, kchunk), is_split(kchunk) ? list_grid(nprob, p, kchunk) : 0);
+}
+
 // Writes descriptors and the work list (problems in the given order, tiles
 // x-fastest, then y, then z chunk; one item per column tile and the shares for
 // the split list) into the workspace, and sets nitems, kchunk and grid.
@@ -196,9 +214,7 @@ int write_workspace(hipStream_t stream, int nprob, const mgad_problem *p, int kc
     const long nitems = count_items(nprob, p, kchunk);
     const bool split = is_split(kchunk);
     const int tw = tile_w(kchunk);
-    long planes_total = 0;
-    for (int i = 0; i < nprob; i++) planes_total += col_tiles(p[i], tw) * p[i].nz;
-    const long grid = std::max(1l, std::min(split ? planes_total : nitems, static_cast<long>(resident_blocks(tw))));
+    const long grid = list_grid(nprob, p, kchunk);
     const long nshares = split ? grid : 0;
     if (ws_bytes(nprob, nitems, nshares) > ws_size || nitems >= (1l << 31)) {
         set_error(MGAD_ERR_INVALID_ARG);
@@ -344,6 +360,40 @@ int init_ctrls(hipStream_t stream, int precision, int nprob, const mgad_problem 
 }  // namespace
 }  // namespace mgad
 
+// The work list of mgad_batch_init_static: one array per GPU, held from
+// library load until the process ends.  Host code never passes its name to a
+// kernel or a copy (on the host it is only a stand-in for the symbol): it uses
+// the device address scratch_ptr() looked up for the current device.
+static __device__ __align__(256) unsigned char scratch[MGAD_BATCH_SCRATCH_BYTES];
+
+namespace mgad {
+namespace {
+
+constexpr size_t kBatchScratchBytes = MGAD_BATCH_SCRATCH_BYTES;
+
+// Device address of `scratch` on the current device, looked up once per
+// device with that device current (each GPU holds its own copy of the array).
+void *scratch_ptr() {
+    constexpr int kMaxDevices = 64;
+    static void *ptrs[kMaxDevices] = {};
+    std::lock_guard<std::recursive_mutex> lock(host_mutex());
+    int dev = -1;
+    if (check_hip(hipGetDevice(&dev))) return nullptr;
+    if (dev < 0 || dev >= kMaxDevices) {
+        set_error(MGAD_ERR_UNSUPPORTED);
+        return nullptr;
+    }
+    if (!ptrs[dev]) {
+        void *p = nullptr;
+        if (check_hip(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)))) return nullptr;
+        ptrs[dev] = p;
+    }
+    return ptrs[dev];
+}
+
+}  // namespace
+}  // namespace mgad
+
 using namespace mgad;
 
 extern "C" size_t mgad_batch_workspace_bytes(int nprob, const mgad_problem *probs) {
@@ -353,18 +403,13 @@ extern "C" size_t mgad_batch_workspace_bytes(int nprob, const mgad_problem *prob
     return ws_bytes(nprob, count_items(nprob, probs, 4), resident_blocks(kBT));
 }
 
-extern "C" int mgad_batch_init(hipStream_t stream, int precision, const mgad_scheme *scheme, int nprob,
-                               const mgad_problem *probs, void *workspace, size_t workspace_bytes, int kchunk,
-                               double t0, double tend, double cfl, mgad_batch *out) {
-    if (!scheme || !out || !workspace || (precision != MGAD_F64 && precision != MGAD_F32) ||
-        !problems_ok(nprob, probs) || (kchunk < 0 && !is_split(kchunk))) {
-        set_error(MGAD_ERR_INVALID_ARG);
-        return MGAD_ERR_INVALID_ARG;
-    }
-    if (!pick_batch<double>(*scheme)) {
-        set_error(MGAD_ERR_UNSUPPORTED);
-        return MGAD_ERR_UNSUPPORTED;
-    }
+namespace {
+
+// mgad_batch_init on a given workspace: kchunk != 0 writes that list, 0
+// autotunes among the five lists.
+int batch_init_ws(hipStream_t stream, int precision, const mgad_scheme *scheme, int nprob,
+                  const mgad_problem *probs, void *workspace, size_t workspace_bytes, int kchunk, double t0,
+                  double tend, double cfl, mgad_batch *out) {
     mgad_batch b{};
     b.workspace = workspace;
     b.workspace_bytes = workspace_bytes;
@@ -389,8 +434,7 @@ extern "C" int mgad_batch_init(hipStream_t stream, int precision, const mgad_sch
         for (int pass = 0; pass < 2 && !st; pass++) {
             for (size_t c = 0; c < cands.size(); c++) {
                 const int kc = cands[c];
-                const long nshares = is_split(kc) ? resident_blocks(tile_w(kc)) : 0;
-                if (ws_bytes(nprob, count_items(nprob, probs, kc), nshares) > workspace_bytes) continue;
+                if (list_bytes(nprob, probs, kc) > workspace_bytes) continue;
                 if ((st = write_workspace(stream, nprob, probs, kc, workspace, workspace_bytes, b))) break;
                 st = init_ctrls(stream, precision, nprob, probs, t0, 1e300, cfl);
                 if (!st) st = batch_stage(stream, b, 1);  // warm-up
@@ -419,6 +463,50 @@ extern "C" int mgad_batch_init(hipStream_t stream, int precision, const mgad_sch
     if ((st = init_ctrls(stream, precision, nprob, probs, t0, tend, cfl))) return st;
     *out = b;
     return MGAD_OK;
+}
+
+bool batch_args_ok(int precision, const mgad_scheme *scheme, int nprob, const mgad_problem *probs, int kchunk,
+                   const mgad_batch *out) {
+    return scheme && out && (precision == MGAD_F64 || precision == MGAD_F32) && problems_ok(nprob, probs) &&
+           (kchunk >= 0 || is_split(kchunk));
+}
+
+}  // namespace
+
+extern "C" int mgad_batch_init(hipStream_t stream, int precision, const mgad_scheme *scheme, int nprob,
+                               const mgad_problem *probs, void *workspace, size_t workspace_bytes, int kchunk,
+                               double t0, double tend, double cfl, mgad_batch *out) {
+    if (!workspace || !batch_args_ok(precision, scheme, nprob, probs, kchunk, out)) {
+        set_error(MGAD_ERR_INVALID_ARG);
+        return MGAD_ERR_INVALID_ARG;
+    }
+    if (!pick_batch<double>(*scheme)) {
+        set_error(MGAD_ERR_UNSUPPORTED);
+        return MGAD_ERR_UNSUPPORTED;
+    }
+    return batch_init_ws(stream, precision, scheme, nprob, probs, workspace, workspace_bytes, kchunk, t0, tend, cfl,
+                         out);
+}
+
+extern "C" int mgad_batch_init_static(hipStream_t stream, int precision, const mgad_scheme *scheme, int nprob,
+                                      const mgad_problem *probs, int kchunk, double t0, double tend, double cfl,
+                                      mgad_batch *out) {
+    if (kchunk == 0 || !batch_args_ok(precision, scheme, nprob, probs, kchunk, out)) {
+        set_error(MGAD_ERR_INVALID_ARG);
+        return MGAD_ERR_INVALID_ARG;
+    }
+    if (!pick_batch<double>(*scheme)) {
+        set_error(MGAD_ERR_UNSUPPORTED);
+        return MGAD_ERR_UNSUPPORTED;
+    }
+    if (list_bytes(nprob, probs, kchunk) > kBatchScratchBytes) {
+        set_error(MGAD_ERR_SCRATCH_FULL);
+        return MGAD_ERR_SCRATCH_FULL;
+    }
+    void *ws = scratch_ptr();
+    if (!ws) return MGAD_ERR_HIP;
+    return batch_init_ws(stream, precision, scheme, nprob, probs, ws, kBatchScratchBytes, kchunk, t0, tend, cfl,
+                         out);
 }
 
 extern "C" int mgad_batch_steps(hipStream_t stream, const mgad_batch *batch, int nsteps, int use_graph) {

@@ -79,6 +79,7 @@ extern "C" const char *mgad_error_string(int code) {
     case MGAD_ERR_UNSUPPORTED: return "unsupported grid shape";
     case MGAD_ERR_HIP: return "HIP runtime error";
     case MGAD_ERR_NONFINITE: return "non-finite max|u|";
+    case MGAD_ERR_SCRATCH_FULL: return "work list larger than the static device scratch";
     default: return "unknown error";
     }
 }

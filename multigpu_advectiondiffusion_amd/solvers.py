@@ -265,10 +265,23 @@ class BurgersBatch:
     def npts(self) -> int:
         return int(sum(nx * ny * nz for nx, ny, nz in self.sizes))
 
-    def set_initial(self, u0s, t0: float = 0.0, tend: float = float("inf"), kchunk: int = 0) -> None:
+    def set_initial(self, u0s, t0: float = 0.0, tend: float = float("inf"), kchunk: int = 0,
+                    static_scratch: bool = False) -> None:
+        """Upload the initial fields and build the work list: in a workspace
+        allocated here (mgad_batch_init; kchunk 0 autotunes), or, with
+        static_scratch, in the library's static device scratch
+        (mgad_batch_init_static, no device allocation; kchunk must name the
+        list, and the batch is valid until the next such call on the device)."""
         for f, u0 in zip(self.fields, u0s):
             f["u"].upload(u0)
         L = _capi.lib()
+        if static_scratch:
+            self.workspace = None
+            rc = L.mgad_batch_init_static(self.stream.handle, self.precision, ctypes.byref(self.scheme),
+                                          len(self.sizes), self.probs, kchunk, t0, tend, self.cfl,
+                                          ctypes.byref(self.batch))
+            _capi.check(rc, "mgad_batch_init_static")
+            return
         nbytes = L.mgad_batch_workspace_bytes(len(self.sizes), self.probs)
         self.workspace = DeviceArray((nbytes,), np.uint8)
         rc = L.mgad_batch_init(self.stream.handle, self.precision, ctypes.byref(self.scheme), len(self.sizes),

@@ -82,3 +82,24 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
     monkeypatch.setattr(_capi, "LIB_PATH", str(tmp_path / "absent.so"))
     with pytest.raises(_capi.MgadError):
         _capi.lib()
+
+
+def test_static_batch_init_rejects_without_device():
+    """mgad_batch_init_static checks its arguments and the work list's size
+    against the static scratch before any device call: kchunk 0 (autotune) is
+    not offered, and a 512^3 problem cut into items of 4 planes needs more
+    than MGAD_BATCH_SCRATCH_BYTES (the field pointers are never touched)."""
+    import ctypes
+
+    L = _capi.lib()
+    fake = _capi.Problem(0x1000, 0x2000, 0x3000, 0x4000, 512, 512, 512, 2 / 512, 2 / 512, 2 / 512)
+    probs = (_capi.Problem * 1)(fake)
+    sc = solvers.scheme_struct("Z2")
+    out = _capi.Batch()
+    args = (None, _capi.F64, ctypes.byref(sc), 1, probs)
+    tail = (0.0, 1.0, 0.4, ctypes.byref(out))
+    assert L.mgad_batch_init_static(*args, 0, *tail) == _capi.MGAD_ERR_INVALID_ARG
+    assert _capi.last_error() == _capi.MGAD_ERR_INVALID_ARG
+    assert L.mgad_batch_init_static(*args, 4, *tail) == _capi.MGAD_ERR_SCRATCH_FULL
+    assert _capi.last_error() == _capi.MGAD_ERR_SCRATCH_FULL
+    assert "scratch" in _capi.error_string(_capi.MGAD_ERR_SCRATCH_FULL)
