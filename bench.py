@@ -136,7 +136,6 @@ def cpu_baseline(n: int, scheme: str, seed: int):
         "build": "gcc -O3 -march=native -fopenmp -ffp-contract=off (oracle/Makefile native)",
         "cpu_model": _cpu_model(),
         "host_cpus": os.cpu_count(),
-        "affinity_cpus": len(os.sched_getaffinity(0)),
         "omp": {k: os.environ.get(k) for k in ("OMP_NUM_THREADS", "OMP_PROC_BIND", "OMP_PLACES")},
     }
 

@@ -374,8 +374,7 @@ int run_steps(int precision, hipStream_t stream, const mgad_scheme *scheme, unsi
     hipGraphExec_t exec = nullptr;
     if (it == cache.end()) {
         if (cache.size() > 32) {
-            for (auto &kv : cache) (void)hipGraphExecDestroy(kv.second);
-            cache.clear();
+            clear_graph_cache(cache);
         }
         hipGraph_t g = nullptr;
         int st = check_hip(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));

@@ -379,8 +379,7 @@ extern "C" int mgad_burgers_ref_steps(hipStream_t stream, const mgad_scheme *sch
     auto it = cache.find(key);
     if (it == cache.end()) {
         if (cache.size() > 16) {
-            for (auto &kv : cache) (void)hipGraphExecDestroy(kv.second);
-            cache.clear();
+            clear_graph_cache(cache);
         }
         hipGraph_t gr = nullptr;
         if ((st = check_hip(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal)))) return st;

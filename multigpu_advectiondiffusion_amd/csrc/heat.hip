@@ -467,8 +467,7 @@ extern "C" int mgad_heat7_fe_steps(hipStream_t stream, unsigned int nx, unsigned
     hipGraphExec_t exec = nullptr;
     if (it == cache.end()) {
         if (cache.size() > 16) {
-            for (auto &kv : cache) (void)hipGraphExecDestroy(kv.second);
-            cache.clear();
+            clear_graph_cache(cache);
         }
         hipGraph_t g = nullptr;
         int st = check_hip(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
@@ -518,8 +517,7 @@ extern "C" int mgad_heat13_rk3_steps(hipStream_t stream, unsigned int nx, unsign
     auto it = cache.find(key);
     if (it == cache.end()) {
         if (cache.size() > 16) {
-            for (auto &kv : cache) (void)hipGraphExecDestroy(kv.second);
-            cache.clear();
+            clear_graph_cache(cache);
         }
         hipGraph_t g = nullptr;
         if ((st = check_hip(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal)))) return st;

@@ -6,8 +6,11 @@
 #include <cstring>
 #include <vector>
 
-#include "mgad_common.hpp"
+#include "../../include/mgad.h"
 
+namespace mgad {
+void set_error(int code);  // host.cpp: the sticky error code
+}
 using mgad::set_error;
 
 namespace {

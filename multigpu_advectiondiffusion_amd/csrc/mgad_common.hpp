@@ -37,6 +37,16 @@ struct AdvParams {
 };
 AdvParams adv_params();
 
+// Destroys every step graph of a full cache.  It first waits for the device,
+// so no exec is destroyed while a replay of it, queued by any host thread on
+// any stream, is pending or running.
+template <class Map>
+void clear_graph_cache(Map &cache) {
+    (void)hipDeviceSynchronize();
+    for (auto &kv : cache) (void)hipGraphExecDestroy(kv.second);
+    cache.clear();
+}
+
 // Diffusion coefficients of CopyToConstantMemory, passed to kernels by value.
 struct DiffCoef { double kx, ky, kz; };
 DiffCoef diff_coef();

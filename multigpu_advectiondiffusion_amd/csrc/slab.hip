@@ -232,8 +232,7 @@ extern "C" int mgad_burgers_slab_steps(hipStream_t stream, const mgad_scheme *sc
     auto it = cache.find(key);
     if (it == cache.end()) {
         if (cache.size() > 16) {
-            for (auto &kv : cache) (void)hipGraphExecDestroy(kv.second);
-            cache.clear();
+            clear_graph_cache(cache);
         }
         hipGraph_t g = nullptr;
         int st = check_hip(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));

@@ -431,8 +431,7 @@ extern "C" int mgad_weno7_steps(hipStream_t stream, unsigned int nx, unsigned in
     auto it = cache.find(key);
     if (it == cache.end()) {
         if (cache.size() > 16) {
-            for (auto &kv : cache) (void)hipGraphExecDestroy(kv.second);
-            cache.clear();
+            clear_graph_cache(cache);
         }
         hipGraph_t gph = nullptr;
         int st = check_hip(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));

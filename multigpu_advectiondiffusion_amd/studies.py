@@ -448,10 +448,11 @@ def weno7_cases(steps: int = 10, warmup: int = 2) -> dict:
     return out
 
 
-def adjoint_trace(n: int = 128, steps: int = 100) -> dict:
+def adjoint_trace(n: int = 128, steps: int = 100, copy_calib: bool = False) -> dict:
     """One store-all adjoint of a `steps`-step n^3 WENO5-Z run: the workload
     of the adjoint kernel trace (forward stages, then per stage k_vjp_xy,
-    k_vjp_z and k_vjp_tail)."""
+    k_vjp_z and k_vjp_tail).  copy_calib: afterwards, three device copies of
+    one n^3 FP64 field (the known-size reference for HBM counters)."""
     u0 = solvers.ic_fourier(n, seed=5)
     s = solvers.PeriodicBurgers(n, scheme="Z2")
     s.set_initial(u0, tend=float("inf"))
@@ -459,6 +460,10 @@ def adjoint_trace(n: int = 128, steps: int = 100) -> dict:
     T = s.state().t
     adj = solvers.BurgersAdjoint(n, scheme="Z2", max_steps=steps + 8, every=steps + 8, stream=s.stream)
     J, _, nst = adj(u0, 0.9 * u0, T)
+    if copy_calib:
+        for _ in range(3):
+            s.w1.copy_from(s.u, s.stream)
+        s.stream.synchronize()
     return {"study": "adjoint_trace", "grid": n, "steps": nst, "J": J}
 
 
@@ -483,6 +488,8 @@ def main(argv=None) -> int:
                              "weno7"])
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="adjoint_trace: three device copies of one field afterwards (counter calibration)")
     args = ap.parse_args(argv)
     if args.study == "batched":
         r = batched(args.steps)
@@ -499,7 +506,7 @@ def main(argv=None) -> int:
     elif args.study == "adjoint":
         r = adjoint(args.n if args.n != 512 else 128)
     elif args.study == "adjoint_trace":
-        r = adjoint_trace(args.n if args.n != 512 else 128, args.steps if args.steps != 200 else 100)
+        r = adjoint_trace(args.n if args.n != 512 else 128, args.steps if args.steps != 200 else 100, args.copy_calib)
     else:
         r = fp32(args.n)
     print(json.dumps(r), flush=True)
