@@ -255,16 +255,17 @@ def main() -> int:
         a = studies.adjoint(n=128)
         # per-direction |FD - AD| / |AD| over the 16 directions, central
         # differences at h = 1e-6 and their Richardson extrapolation
-        pick = ("max_rel_err", "median_rel_err")
+        pick = ("max_rel_err", "median_rel_err", "max_err_over_grad_norm")
+        times = ("T0.02", "T0.2", "T0.5")
         out["adjoint"] = {"cost_ratio": a["cost_ratio"], "steps": a["cost_steps"], "grid": 128,
                           "checkpoint_every": a["sqrtN"]["checkpoint_every"],
                           "cost_ratio_store_all": a["store_all"]["cost_ratio"],
                           "fd_central_h1e-6": {T: {k: a[T]["central"]["h1e-06"][k] for k in pick}
-                                               for T in ("T0.2", "T0.5")},
+                                               for T in times},
                           "fd_richardson_h1e-6": {T: {k: a[T]["richardson"]["h1e-06"][k] for k in pick}
-                                                  for T in ("T0.2", "T0.5")},
+                                                  for T in times},
                           "fd_central_max_rel_err_by_h": {T: {h: v["max_rel_err"] for h, v in a[T]["central"].items()}
-                                                          for T in ("T0.2", "T0.5")}}
+                                                          for T in times}}
         print("adjoint study done", file=sys.stderr, flush=True)
         r = studies.reference_cases()
         out["reference_cases"] = [{k: c[k] for k in ("grid", "steps", "fused_gpts", "published_gpts",
