@@ -138,25 +138,40 @@ __device__ __forceinline__ double div_inrange(double a, double b) {
     return __builtin_fma(rem, r, q);
 }
 
-// When the weight divisions of weno_side_s may use div_inrange (its FD
-// variant) with results identical to IEEE division.  With max|u| <= 2^40 over
-// the stage input (and |alpha| <= 2^40 for a global LF split, checked here;
+// When the divisions of weno_side_s may use div_inrange (its FD variant)
+// with results identical to IEEE division.  With max|u| <= 2^40 over the
+// stage input (and |alpha| <= 2^40 for a global LF split, checked here;
 // callers check |u|) every split flux has |f| < 2^80, so each 13/12 s^2 < 2^165
 // and 1/4 q^2 < 2^164: 0 <= beta_k < 2^166.
-//   WENO-Z, 2^-300 <= eps <= 1: b = beta+eps is in [2^-300, 2^167] and
-//     a = tau5 < 2^167.  For tau5 = 0 or tau5 >= 2^-380 the ranges above
-//     hold.  For 0 < tau5 < 2^-380 both quotients are below 2^-79, so
-//     1 + q (p=1) and 1 + q^2 (p=2) round to exactly 1 either way and the
-//     weight is d_k in both.
+//   WENO-Z, 2^-150 <= eps <= 1:
+//   - weights: b = beta+eps is in [2^-150, 2^167] and a = tau5 < 2^167.  For
+//     tau5 = 0 or tau5 >= 2^-380 the ranges of div_inrange hold.  For
+//     0 < tau5 < 2^-380 both quotients are below 2^-229, so 1 + q (p=1) and
+//     1 + q^2 (p=2) round to exactly 1 either way and the weight is d_k in both.
+//   - final division num/(6 sum): q = tau5/(beta+eps) < 2^317, so each
+//     alpha_k = d_k (1 + q^p) < 2^634 and alpha_1 >= d_1 = 0.6: 6 sum is in
+//     [3.6, 2^637].  The candidates are below 2^85 in magnitude, so
+//     |num| < 2^721.  For |num| >= 2^-121 (checked per node) the exponent gap
+//     is at most 757 and div_inrange applies; smaller numerators, zeros of
+//     either sign and NaN take the IEEE division.
+//   - the shared-memory rounding's 1/sum: a = 1, b = sum in [0.6, 2^635].
 //   WENO-JS, 2^-190 <= eps <= 1: (eps+beta)^2 is in [2^-380, 2^334], so
 //     alpha_k = d_k/(..)^2 is in [2^-338, 2^380] and alpha_k/sum in
 //     [2^-720, 1].
-// The final division of WENO-Z (numerator unbounded below) stays IEEE.  A NaN
-// or infinite alpha fails the test.
+// A NaN or infinite alpha fails the test.
 template <int WENO>
 __host__ __device__ inline bool fast_division_ok(double alpha, double eps) {
-    const double eps_lo = WENO == kWenoJS ? 0x1p-190 : 0x1p-300;
+    const double eps_lo = WENO == kWenoJS ? 0x1p-190 : 0x1p-150;
     return fabs(alpha) <= 0x1p40 && eps >= eps_lo && eps <= 1.0;
+}
+
+// num / den of the final WENO-Z division under FD (fast_division_ok holds):
+// div_inrange where |num| >= 2^-121, IEEE otherwise (a branch every lane of a
+// wave skips in the common case).
+__device__ __forceinline__ double div_final(double num, double den) {
+    double r = div_inrange(num, den);
+    if (__builtin_expect(!(fabs(num) >= 0x1p-121), 0)) r = num / den;
+    return r;
 }
 
 // WENO-Z weight d*(1 + (tau5/(beta+eps))^P), kernels.cu:163-165 (P = 1);
@@ -202,8 +217,10 @@ __device__ __forceinline__ double weno_side_s(double v0, double v1, double v2, d
         const double a1 = zweight<P, FD>(d1, tau5, B1, eps);
         const double a2 = zweight<P, FD>(d2, tau5, B2, eps);
         const double sum = (a0 + a1) + a2;
-        if (WENO == kWenoZB5S) return ((a0 * c0 + a1 * c1) + a2 * c2) * (kC1o6 * (1.0 / sum));
-        return ((a0 * c0 + a1 * c1) + a2 * c2) / (6.0 * sum);
+        if (WENO == kWenoZB5S)
+            return ((a0 * c0 + a1 * c1) + a2 * c2) * (kC1o6 * (FD ? div_inrange(1.0, sum) : 1.0 / sum));
+        const double num = (a0 * c0 + a1 * c1) + a2 * c2;
+        return FD ? div_final(num, 6.0 * sum) : num / (6.0 * sum);
     } else {
         const double e0 = (eps + B0) * (eps + B0);
         const double e1 = (eps + B1) * (eps + B1);

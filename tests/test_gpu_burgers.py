@@ -60,9 +60,15 @@ def test_stages_bitwise(hipdev, scheme, shape):
 
 
 # The stage kernels shorten the weight divisions only when max|u| <= 2^40 and
-# eps is in range (2^-300..1 for Z, 2^-190..1 for JS); these cases fall
-# outside and take the IEEE sequence, which must match the oracle as well.
-@pytest.mark.parametrize("case", ["z_eps_1e-95", "js_eps_1e-100", "z_u_times_2^41"])
+# eps is in range (2^-150..1 for Z, 2^-190..1 for JS); the first four cases
+# fall outside and take the IEEE sequence, which must match the oracle as
+# well.  Within range, the final WENO-Z division is shortened only for
+# numerators of magnitude >= 2^-121: "z_tiny" (|u| ~ 1e-20, every numerator
+# ~1e-40) takes the IEEE division on every node, "z_zero_half" (u = 0 on half
+# the planes: zero numerators there, ordinary ones elsewhere) on some nodes
+# of a wave and not others.
+@pytest.mark.parametrize("case", ["z_eps_1e-95", "z_eps_1e-60", "js_eps_1e-100", "z_u_times_2^41", "z_tiny",
+                                  "z_zero_half"])
 def test_stages_ieee_division_path_bitwise(hipdev, case):
     from multigpu_advectiondiffusion_amd import _capi
 
@@ -72,8 +78,15 @@ def test_stages_ieee_division_path_bitwise(hipdev, case):
     u0 = solvers.ic_fourier(shape, seed=9)
     if case == "z_eps_1e-95":
         eps = 1e-95
+    elif case == "z_eps_1e-60":
+        eps = 1e-60
     elif case == "js_eps_1e-100":
         eps = 1e-100
+    elif case == "z_tiny":
+        u0 = u0 * 1e-20
+    elif case == "z_zero_half":
+        u0 = u0.copy()
+        u0[: shape[2] // 2] = 0.0
     else:
         u0 = u0 * 2.0 ** 41
     orc = dict(weno=O.WENO_JS if scheme == "JS" else O.WENO_Z, p=p, eps=eps, lf=O.LF_GLOBAL)
