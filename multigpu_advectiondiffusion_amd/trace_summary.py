@@ -24,7 +24,8 @@ FETCH_SIZE and WRITE_SIZE (KiB) are scaled by what they read for the last C
 factor = K / mean counter of those copies; the scaled figure is in bytes.
 --skip-idle leaves out, per kernel and counter, dispatches that read under 5 %
 of that kernel's largest value of the counter: stage launches that found the
-end time already reached and returned at once.
+end time already reached and returned at once.  --grid N keeps only the
+dispatches of N threads (one problem size of a run that launches several).
 """
 from __future__ import annotations
 
@@ -85,7 +86,7 @@ def read_counters(paths):
                 k["shape"] = {key: r.get(key) for key in ("Grid_Size", "Workgroup_Size", "VGPR_Count",
                                                           "Accum_VGPR_Count", "LDS_Block_Size", "Scratch_Size")}
                 did = (path, int(r["Dispatch_Id"]))
-                d = k["dispatches"].setdefault(did, {})
+                d = k["dispatches"].setdefault(did, {"_grid": int(r["Grid_Size"])})
                 c = r["Counter_Name"]
                 d[c] = d.get(c, 0.0) + float(r["Counter_Value"])
     return out
@@ -105,7 +106,7 @@ def copy_factors(kernels, calib_kib: float, calib_n: int):
     return fac
 
 
-def pmc_report(paths, kernel_rx=None, points=0, calib_kib=0.0, calib_n=3, skip_idle=False) -> str:
+def pmc_report(paths, kernel_rx=None, points=0, calib_kib=0.0, calib_n=3, skip_idle=False, grid=0) -> str:
     kernels = read_counters(paths)
     fac = copy_factors(kernels, calib_kib, calib_n)
     lines = []
@@ -115,13 +116,15 @@ def pmc_report(paths, kernel_rx=None, points=0, calib_kib=0.0, calib_n=3, skip_i
     for name, k in kernels.items():
         if name == COPY_KERNEL or (rx and not rx.search(name)):
             continue
-        disp = list(k["dispatches"].values())
-        sh = k["shape"]
+        disp = [d for d in k["dispatches"].values() if not grid or d["_grid"] == grid]
+        if not disp:
+            continue
+        sh = dict(k["shape"], Grid_Size=disp[-1]["_grid"])
         lines.append(f"\n{name}")
         lines.append(f"  dispatches {len(disp)}; grid {sh['Grid_Size']} threads, workgroup {sh['Workgroup_Size']}, "
                      f"VGPR {sh['VGPR_Count']} (+{sh['Accum_VGPR_Count']} AGPR), LDS {sh['LDS_Block_Size']} B, "
                      f"scratch {sh['Scratch_Size']} B")
-        ctrs = sorted({c for d in disp for c in d})
+        ctrs = sorted({c for d in disp for c in d if c != "_grid"})
         for c in ctrs:
             vals = [d[c] for d in disp if c in d]
             if skip_idle:
@@ -147,9 +150,10 @@ def main(argv=None) -> int:
     ap.add_argument("--calib-kib", type=float, default=0.0, help="KiB each calibration copy moves (--pmc)")
     ap.add_argument("--calib-n", type=int, default=3, help="number of calibration copies, the last ones (--pmc)")
     ap.add_argument("--skip-idle", action="store_true", help="drop dispatches under 5%% of the kernel's largest value")
+    ap.add_argument("--grid", type=int, default=0, help="only dispatches of this grid size in threads (--pmc)")
     args = ap.parse_args(argv)
     if args.pmc:
-        print(pmc_report(args.files, args.kernel, args.points, args.calib_kib, args.calib_n, args.skip_idle))
+        print(pmc_report(args.files, args.kernel, args.points, args.calib_kib, args.calib_n, args.skip_idle, args.grid))
         return 0
     if len(args.files) != 1:
         ap.error("a kernel trace summary reads one file")
