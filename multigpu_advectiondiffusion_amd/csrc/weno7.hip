@@ -47,16 +47,44 @@ __device__ __forceinline__ void w7_beta(double a, double b, double c, double d, 
 }
 
 // w_k = alpha_k / sum, alpha_k = g_k/(eps + B_k)^2, eps = 1e-6 (:35-47).
+// Where all four divisors d_k = (eps + B_k)^2 lie in [2^-370, 2^370] the eight
+// divisions are the shortened div_inrange, equal to IEEE division there: the
+// numerators g_k are in [1/35, 18/35], so alpha_k is in [2^-376, 2^370], the
+// sum in [2^-376, 2^372], and every quotient's operands and exponent gap
+// (at most 749) are within div_inrange's range.  The four normalisations
+// share one divisor, so they share its reciprocal.  Other nodes (NaN, a
+// divisor near zero or beyond 2^370: |u| above ~2^41) take IEEE division.
+// The range test reads the high words only: the divisors are squares (sign
+// bit clear, NaN aside), whose bit patterns order as their values, so
+// d in [2^-370, 2^370) <=> hi(d) in [hi(2^-370), hi(2^370)); a NaN of either
+// sign has a high word above hi(2^370).
+constexpr unsigned kW7DivLo = 0x28D00000u;  // high word of 2^-370 (biased exponent 653)
+constexpr unsigned kW7DivHi = 0x57100000u;  // high word of 2^370 (biased exponent 1393)
+
+__device__ __forceinline__ unsigned w7_hi(double d) { return static_cast<unsigned>(__double2hiint(d)); }
+
 __device__ __forceinline__ void w7_weights(const double B[4], double g0, double g1, double g2, double g3,
                                            double w[4]) {
     const double eps = 1e-6;
-    const double a0 = g0 / ((eps + B[0]) * (eps + B[0])), a1 = g1 / ((eps + B[1]) * (eps + B[1])),
-                 a2 = g2 / ((eps + B[2]) * (eps + B[2])), a3 = g3 / ((eps + B[3]) * (eps + B[3]));
+    const double d0 = (eps + B[0]) * (eps + B[0]), d1 = (eps + B[1]) * (eps + B[1]),
+                 d2 = (eps + B[2]) * (eps + B[2]), d3 = (eps + B[3]) * (eps + B[3]);
+    const unsigned h0 = w7_hi(d0), h1 = w7_hi(d1), h2 = w7_hi(d2), h3 = w7_hi(d3);
+    const unsigned hmin = min(min(h0, h1), min(h2, h3)), hmax = max(max(h0, h1), max(h2, h3));
+    const double a0 = div_inrange(g0, d0), a1 = div_inrange(g1, d1), a2 = div_inrange(g2, d2),
+                 a3 = div_inrange(g3, d3);
     const double sum = a0 + a1 + a2 + a3;
-    w[0] = a0 / sum;
-    w[1] = a1 / sum;
-    w[2] = a2 / sum;
-    w[3] = a3 / sum;
+    w[0] = div_inrange(a0, sum);
+    w[1] = div_inrange(a1, sum);
+    w[2] = div_inrange(a2, sum);
+    w[3] = div_inrange(a3, sum);
+    if (__builtin_expect(!(hmin >= kW7DivLo && hmax < kW7DivHi), 0)) {
+        const double b0 = g0 / d0, b1 = g1 / d1, b2 = g2 / d2, b3 = g3 / d3;
+        const double bs = b0 + b1 + b2 + b3;
+        w[0] = b0 / bs;
+        w[1] = b1 / bs;
+        w[2] = b2 / bs;
+        w[3] = b3 / bs;
+    }
 }
 
 constexpr double kR12 = 1.0 / 12;  // RN(1/12) for div_by

@@ -26,6 +26,29 @@ def test_rhs_bitwise(hipdev, shape):
     assert np.array_equal(got, ref), np.argwhere(got != ref)[:5]
 
 
+# The weights take the shortened division where every divisor (eps + B_k)^2
+# lies in [2^-370, 2^370]: "huge" (|u| near 2^44 on half the planes, divisors
+# near 2^386 there) sends those nodes to IEEE division, "zeros" gives
+# divisors eps^2 on the zero half.
+@pytest.mark.parametrize("case", ["huge", "zeros"])
+def test_rhs_bitwise_division_paths(hipdev, case):
+    shape = (24, 20, 18)
+    s = solvers.Weno7Burgers(shape)
+    rng = np.random.default_rng(11)
+    u = s.gaussian() + 0.05 * rng.standard_normal(s.gaussian().shape)
+    if case == "huge":
+        u[: shape[2] // 2] *= 2.0 ** 44
+    else:
+        u[: shape[2] // 2] = 0.0
+    du, dL = DeviceArray.from_host(u), DeviceArray(u.shape)
+    s.rhs(du, dL)
+    s.stream.synchronize()
+    got = dL.download()
+    ref = O.weno7_rhs(u, (s.dx, s.dy, s.dz))
+    assert np.isfinite(ref).all()
+    assert np.array_equal(got, ref), np.argwhere(got != ref)[:5]
+
+
 @pytest.mark.parametrize("graph", [False, True])
 def test_steps_bitwise(hipdev, graph):
     shape = (40, 36, 28)
