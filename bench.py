@@ -281,8 +281,10 @@ def main() -> int:
             out[f"burgers_{sch.lower()}_256_to_t0.5"] = run_to_half(sch)
         print("runs to t = 0.5 done", file=sys.stderr, flush=True)
         sl = studies.slab_cases()
-        out["z_slabs"] = {k: sl[k] for k in ("one_slab_256x256x32_gpts", "eight_slabs_256_gpts", "unsplit_256_gpts",
-                                             "one_slab_vs_unsplit", "eight_slabs_vs_unsplit")}
+        out["z_slabs"] = {k: sl[k] for k in ("one_slab_256x256x32_gpts", "one_slab_256x256x32_tuned_gpts",
+                                             "one_slab_tuned_tile_kchunk", "eight_slabs_256_gpts", "unsplit_256_gpts",
+                                             "one_slab_vs_unsplit", "one_slab_tuned_vs_unsplit",
+                                             "eight_slabs_vs_unsplit")}
         print("slab study done", file=sys.stderr, flush=True)
         w7 = studies.weno7_cases()
         out["weno7_256"] = w7["grid_256"]

@@ -28,8 +28,9 @@ adjoint: J = 1/2 |u(T) - u_target|^2 for WENO5-Z (p=2) Burgers at n^3 FP64,
     gradient (forward with sqrt(N) checkpoints + recompute + reverse) over
     the plain forward run, on the run of 400 steps.  Accuracy: the gradient
     along 16 seeded unit directions against central finite differences
-    (formed point-wise) at T = 0.2 and T = 0.5 (shocks form near t = 0.03
-    for this field), for h in {1e-5, 3e-6, 2e-6, 1e-6, 5e-7, 3e-7, 1e-7}, and
+    (formed point-wise) at T = 0.02 (before the first shock, which forms
+    near t = 0.03-0.04 for this field), 0.2 and 0.5, for h in
+    {1e-5, 3e-6, 2e-6, 1e-6, 5e-7, 3e-7, 1e-7}, and
     against the Richardson extrapolation (4 D(h/2) - D(h))/3 at h = 2e-6 and
     1e-6; per direction the relative error |FD - AD| / |AD|.
 reference: the reference's own viscous Burgers runs (512^3 and 1000x1000x200
@@ -39,13 +40,16 @@ reference: the reference's own viscous Burgers runs (512^3 and 1000x1000x200
 slab: the headline WENO5-Z (p=2) FP64 run on z slabs with 3 ghost planes:
     one GPU's share of an 8-way 256^3 decomposition (one 256x256x32 slab
     whose exchange wraps onto itself) and all 8 slabs of 256^3 on one GPU,
-    against the unsplit 256^3 run; each 30 hipGraph-replayed steps after 3
-    warm-up steps.  The exchange copies stand in for the per-stage neighbour
-    send/recv.
+    against the unsplit 256^3 run; each at least 30 hipGraph-replayed steps
+    and 0.2 s after 0.3 s of warm-up steps.  The exchange copies stand in for
+    the per-stage neighbour send/recv.
 weno7: the WENO7 Burgers prototype (LFWENO7FDM3d.m: Gaussian initial field,
-    CFL 0.4, low-storage RK45) at 256^3 FP64, 10 steps after 2 warm-up steps,
-    hipGraph-replayed; the prototype's own 100^3 run to tEnd 0.4 with its step
-    count and wall time.
+    CFL 0.4, low-storage RK45) at 256^3 FP64, at least 10 steps and 0.2 s
+    after 0.3 s of warm-up steps, hipGraph-replayed; the prototype's own 100^3
+    run to tEnd 0.4 with its step count and wall time.
+The GPU's clock rises over the first ~50 ms of sustained work (stage launches
+shorten by up to 30 %), so every timed region above follows warm-up work of
+at least 0.3 s.
 Each prints one JSON line.
 """
 from __future__ import annotations
@@ -57,6 +61,23 @@ import sys
 import numpy as np
 
 from . import hip, solvers
+
+WARM_S = 0.3  # warm-up wall time before a timed region
+MIN_TIMED_S = 0.2  # shortest timed region of the slab and WENO7 studies
+
+
+def warm(step, seconds: float = WARM_S) -> int:
+    """Runs step() (then a device synchronisation) until `seconds` of wall
+    time have passed: the GPU clock ramps up over the first ~50 ms of
+    sustained work.  Returns the number of calls."""
+    import time
+
+    n, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        step()
+        hip.synchronize()
+        n += 1
+    return n
 
 
 class SplitMix64:
@@ -89,7 +110,8 @@ def batched(steps: int = 200, nprob: int = 64, scheme: str = "Z2") -> dict:
 
     def timed(fn):
         b.set_initial(u0s, tend=float("inf"), kchunk=0)
-        fn(2)  # warm-up (also builds the graph once)
+        fn(2)  # builds the graph once
+        warm(lambda: fn(4))
         b.set_initial(u0s, tend=float("inf"), kchunk=b.batch.kchunk)
         hip.synchronize()
         e0.record(b.stream)
@@ -404,22 +426,31 @@ def heat_cases(dropin_steps: int = 5) -> dict:
     return out
 
 
-def slab_cases(steps: int = 30, warmup: int = 3) -> dict:
+def slab_cases(steps: int = 30) -> dict:
     e0, e1 = hip.Event(), hip.Event()
 
     def rate(solver, npts):
         solver.set_initial(solvers.ic_fourier((solver.nx, solver.ny, solver.nz), seed=3))
-        solver.steps(warmup)
+        calls = warm(lambda: solver.steps(5))
+        # at least `steps` steps and MIN_TIMED_S (the warm-up ran 5 * calls steps in WARM_S)
+        k = max(steps, int(5 * calls * MIN_TIMED_S / WARM_S) + 1)
         hip.synchronize()
         e0.record(solver.stream)
-        solver.steps(steps)
+        solver.steps(k)
         e1.record(solver.stream)
         e1.synchronize()
-        return npts * steps / (e0.elapsed_ms(e1) / 1e3) / 1e9
+        return npts * k / (e0.elapsed_ms(e1) / 1e3) / 1e9
 
     out = {"study": "slab", "scheme": "WENO5-Z p=2 eps=1e-40", "steps": steps}
     one = solvers.SlabBurgers((256, 256, 32), 1)
     out["one_slab_256x256x32_gpts"] = rate(one, 256 * 256 * 32)
+    # the slab launch reads the launch table entry of the unsplit grid of its
+    # shape: tune that entry, then time the slab again
+    p = solvers.PeriodicBurgers((256, 256, 32), stream=one.stream)
+    p.set_initial(solvers.ic_fourier((256, 256, 32), seed=3))
+    out["one_slab_tuned_tile_kchunk"] = list(solvers.autotune(p))
+    del p
+    out["one_slab_256x256x32_tuned_gpts"] = rate(one, 256 * 256 * 32)
     del one
     eight = solvers.SlabBurgers((256, 256, 256), 8)
     out["eight_slabs_256_gpts"] = rate(eight, 256 ** 3)
@@ -427,23 +458,25 @@ def slab_cases(steps: int = 30, warmup: int = 3) -> dict:
     whole = solvers.PeriodicBurgers(256)
     out["unsplit_256_gpts"] = rate(whole, 256 ** 3)
     out["one_slab_vs_unsplit"] = out["one_slab_256x256x32_gpts"] / out["unsplit_256_gpts"]
+    out["one_slab_tuned_vs_unsplit"] = out["one_slab_256x256x32_tuned_gpts"] / out["unsplit_256_gpts"]
     out["eight_slabs_vs_unsplit"] = out["eight_slabs_256_gpts"] / out["unsplit_256_gpts"]
     return out
 
 
-def weno7_cases(steps: int = 10, warmup: int = 2) -> dict:
+def weno7_cases(steps: int = 10) -> dict:
     e0, e1 = hip.Event(), hip.Event()
     out = {"study": "weno7", "scheme": "WENO7 (Balsara-Shu forms, eps 1e-6), local LF, LSRK(5,4)"}
     s = solvers.Weno7Burgers(256)
-    s.set_initial(s.gaussian())
-    s.steps(warmup)
+    s.set_initial(s.gaussian(), tend=float("inf"))
+    calls = warm(lambda: s.steps(2))
+    k = max(steps, int(2 * calls * MIN_TIMED_S / WARM_S) + 1)
     hip.synchronize()
     e0.record(s.stream)
-    s.steps(steps)
+    s.steps(k)
     e1.record(s.stream)
     e1.synchronize()
     t = e0.elapsed_ms(e1) / 1e3
-    out["grid_256"] = {"steps": steps, "gpts": 256 ** 3 * steps / t / 1e9, "ms_per_step": 1e3 * t / steps,
+    out["grid_256"] = {"steps": k, "gpts": 256 ** 3 * k / t / 1e9, "ms_per_step": 1e3 * t / k,
                        "stage_launches_per_step": 5}
     del s
     s = solvers.Weno7Burgers(100, cfl=0.4)
