@@ -6,12 +6,10 @@
 //    problem, so items cost about the same whatever the problem size; a grid
 //    of as many 256-thread blocks as the device keeps resident walks the list
 //    (block b takes items b, b+G, ...);
-//  - split: every item is a column tile (16x16 or 32x16) of one problem over
-//    a band of up to 32 planes, items ordered band by band (all tiles of a
-//    band, then the next band); the items' planes, in list order, are cut into
-//    one equal share per resident block, so every block marches the same
-//    number of tile planes and no block waits on a partly filled last round
-//    of items.
+//  - split: every item is a whole 16x16 column tile of one problem, and the
+//    items' planes, in list order, are cut into one equal share per resident
+//    block, so every block marches the same number of tile planes and no
+//    block waits on a partly filled last round of items.
 // Block ids are remapped so that the blocks of one XCD take adjacent items.
 // Each piece of an item runs the same stage body as the single-problem
 // kernel, with the problem's own dt, LF alpha and max|u| slot (its own ctrl),
@@ -82,10 +80,8 @@ __global__ __launch_bounds__(kBT *kBT, 4) void k_burgers_batch(const BatchProble
     }
 }
 
-// Split list: item i is column tile item_tile[i] % ntile of its problem over
-// the band of planes starting at (item_tile[i] / ntile) * g.kchunk (g.kchunk
-// = the band length, the last band shorter), and the items' planes, taken in
-// list order, are cut into G shares of equal plane count;
+// Split list: item i is a whole column tile (all nz planes) and the items'
+// planes, taken in list order, are cut into G shares of equal plane count;
 // block b (remapped as above) runs share s from shares[s] to shares[s + 1]
 // (item, plane) pairs, shares[G] = (nitems, 0).  Every block does the same
 // number of tile planes, whatever the mix of problem sizes.  Tiles are TX x 16
@@ -101,12 +97,9 @@ __global__ __launch_bounds__(TX *kBT, 4) void k_burgers_batch_split(const BatchP
     const uint2 a = shares[s], e = shares[s + 1];
     for (unsigned i = a.x; i <= e.x && i < static_cast<unsigned>(nitems); i++) {
         const BatchProblem &bp = probs[item_prob[i]];
-        const unsigned ntile = static_cast<unsigned>((bp.g.nx + TX - 1) / TX) * static_cast<unsigned>(bp.g.ny / kBT);
-        const int zb = static_cast<int>(item_tile[i] / ntile) * bp.g.kchunk;
-        const int ze = min(zb + bp.g.kchunk, bp.g.nz);
-        const int z0 = i == a.x ? zb + static_cast<int>(a.y) : zb;
-        const int z1 = i == e.x ? zb + static_cast<int>(e.y) : ze;
-        if (z0 < z1) batch_piece<T, WENO, P, TX>(bp, item_tile[i] % ntile, z0, z1, eps, stage, sm);
+        const int z0 = i == a.x ? static_cast<int>(a.y) : 0;
+        const int z1 = i == e.x ? static_cast<int>(e.y) : bp.g.nz;
+        if (z0 < z1) batch_piece<T, WENO, P, TX>(bp, item_tile[i], z0, z1, eps, stage, sm);
     }
 }
 
@@ -135,11 +128,6 @@ SplitKernel<T> pick_split(const mgad_scheme &sc) {
 }
 
 bool is_split(int kchunk) { return kchunk == MGAD_BATCH_SPLIT || kchunk == MGAD_BATCH_SPLIT_WIDE; }
-// z planes per item of a split list: items run band by band, all column tiles
-// of a band in turn, so blocks that run at the same time hold neighbouring
-// tiles at about the same planes
-constexpr int kSplitBand = 32;
-int split_bands(const mgad_problem &p) { return static_cast<int>((p.nz + kSplitBand - 1) / kSplitBand); }
 // tile width of a work list: 32 for the wide split list, else 16
 int tile_w(int kchunk) { return kchunk == MGAD_BATCH_SPLIT_WIDE ? 2 * kBT : kBT; }
 // column tiles of one problem in a list of tile width tw
@@ -158,7 +146,7 @@ size_t ws_bytes(int nprob, long nitems, long nshares) {
 long count_items(int nprob, const mgad_problem *p, int kchunk) {
     long n = 0;
     for (int i = 0; i < nprob; i++)
-        n += col_tiles(p[i], tile_w(kchunk)) * (is_split(kchunk) ? split_bands(p[i]) : (p[i].nz + kchunk - 1) / kchunk);
+        n += col_tiles(p[i], tile_w(kchunk)) * (is_split(kchunk) ? 1 : (p[i].nz + kchunk - 1) / kchunk);
     return n;
 }
 
@@ -243,14 +231,13 @@ int write_workspace(hipStream_t stream, int nprob, const mgad_problem *p, int kc
         bp[i].w1 = p[i].w1;
         bp[i].w2 = p[i].w2;
         bp[i].ctrl = static_cast<Ctrl *>(p[i].ctrl);
-        const int band = split ? kSplitBand : kchunk;
-        bp[i].g = make_grid(p[i].nx, p[i].ny, p[i].nz, p[i].dx, p[i].dy, p[i].dz, band);
-        const long ct = col_tiles(p[i], tw);
-        const long nt = ct * ((p[i].nz + band - 1) / band);
+        bp[i].g = make_grid(p[i].nx, p[i].ny, p[i].nz, p[i].dx, p[i].dy, p[i].dz,
+                            split ? static_cast<int>(p[i].nz) : kchunk);
+        const long nt = col_tiles(p[i], tw) * (split ? 1 : (p[i].nz + kchunk - 1) / kchunk);
         for (long t = 0; t < nt; t++, k++) {
             ip[k] = static_cast<unsigned>(i);
             it[k] = static_cast<unsigned>(t);
-            if (split) planes.push_back(std::min<unsigned>(band, p[i].nz - static_cast<unsigned>(t / ct) * band));
+            if (split) planes.push_back(p[i].nz);
         }
     }
     if (split) cut_shares(planes, nshares, reinterpret_cast<uint2 *>(it + nitems));
