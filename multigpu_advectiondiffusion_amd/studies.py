@@ -310,7 +310,8 @@ def reference_cases(cases=REFERENCE_CASES, dropin_steps: int = 3) -> dict:
         u0 = fileio.init3d_burgers(3, (nx, ny, nz), dx, dy, dz)
         s = solvers.ReferenceBurgers((nx, ny, nz), dx, dy, dz, scheme="Z1", viscous=True)
         s.set_initial(u0)
-        s.steps(1, dt)  # warm-up (graph capture)
+        s.steps(1, dt)  # graph capture
+        warm(lambda: s.steps(2, dt))
         s.set_initial(u0)
         hip.synchronize()
         e0.record(s.stream)
@@ -333,7 +334,7 @@ def reference_cases(cases=REFERENCE_CASES, dropin_steps: int = 3) -> dict:
             R.set_adv_scheme()
             du, duo, dLu = DeviceArray.from_host(u0), DeviceArray(u0.shape), DeviceArray(u0.shape)
             dLu.zero()
-            R.burgers_reference_step(du, duo, dLu, nx, ny, nz, dx, dy, dz, dt)  # warm-up
+            warm(lambda: R.burgers_reference_step(du, duo, dLu, nx, ny, nz, dx, dy, dz, dt))
             hip.synchronize()
             e0.record(None)
             for _ in range(dropin_steps):
@@ -374,14 +375,15 @@ def heat_cases(dropin_steps: int = 5) -> dict:
 
     # what a plain device-to-device copy of one 512^3 field moves (read + write)
     a, b = DeviceArray((512, 512, 512)), DeviceArray((512, 512, 512))
-    b.copy_from(a)
+    warm(lambda: b.copy_from(a))
     t = timed(None, lambda: [b.copy_from(a) for _ in range(10)])
     out["device_copy_512"] = {"GBs": 2 * 8.0 * 512 ** 3 * 10 / t / 1e9}
     del a, b
     for n, steps in ((64, 100), (512, 20)):
         h = solvers.Heat7(n)
         h.set_initial(solvers.ic_gauss_noise(n, seed=1))
-        h.steps(steps)  # warm-up (graph capture)
+        h.steps(steps)  # graph capture
+        warm(lambda: h.steps(steps))
         t = timed(h.stream, lambda: h.steps(steps))
         gbs = 16.0 * n ** 3 * steps / t / 1e9
         out[f"heat7_fe_{n}"] = {"steps": steps, "gpts": n ** 3 * steps / t / 1e9, "algorithmic_GBs": gbs,
@@ -395,7 +397,8 @@ def heat_cases(dropin_steps: int = 5) -> dict:
     u0 = fileio.init3d_heat(3, (nx, ny, nz), dx, dy, dz)
     h = solvers.ReferenceHeat((nx, ny, nz))
     h.set_initial(u0)
-    h.steps(1, dt)  # warm-up
+    h.steps(1, dt)  # graph capture
+    warm(lambda: h.steps(20, dt))
     h.set_initial(u0)
     t = timed(h.stream, lambda: h.steps(iters, dt))
     u = h.solution()
@@ -407,7 +410,7 @@ def heat_cases(dropin_steps: int = 5) -> dict:
             "final_time": 0.1 + dt * iters, "error_l1": l1, "error_l2": l2, "error_linf": linf}
     du, duo, dLu = DeviceArray.from_host(u0), DeviceArray(u0.shape), DeviceArray(u0.shape)
     dLu.zero()
-    R.heat_reference_step(du, duo, dLu, nx, ny, nz, dt)  # warm-up
+    warm(lambda: R.heat_reference_step(du, duo, dLu, nx, ny, nz, dt))
     td = timed(None, lambda: [R.heat_reference_step(du, duo, dLu, nx, ny, nz, dt) for _ in range(dropin_steps)])
     case.update({"dropin_gpts": npts * dropin_steps / td / 1e9,
                  "fused_vs_dropin": (npts * iters / t) / (npts * dropin_steps / td)})
@@ -418,7 +421,8 @@ def heat_cases(dropin_steps: int = 5) -> dict:
     n, steps = 512, 20
     h = solvers.ReferenceHeat((n, n, n))
     h.set_initial(solvers.ic_gauss_noise(n, seed=1))
-    h.steps(1, 1e-6)  # warm-up (graph capture)
+    h.steps(1, 1e-6)  # graph capture
+    warm(lambda: h.steps(2, 1e-6))
     t = timed(h.stream, lambda: h.steps(steps, 1e-6))
     out["heat13_rk3_512"] = {"steps": steps, "gpts": n ** 3 * steps / t / 1e9,
                              "algorithmic_GBs": 64.0 * n ** 3 * steps / t / 1e9,
