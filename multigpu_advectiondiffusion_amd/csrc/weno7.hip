@@ -176,8 +176,10 @@ struct W7Smem {
 };
 
 // MODE 0: one LSRK stage s (res and u_out updated); MODE 1: res = L only.
+// At least 3 waves per SIMD (143 VGPRs, no spills): 4 waves would cap the
+// kernel at 128 VGPRs and spill ~14 values per plane.
 template <int TX, int TY, int MODE>
-__global__ __launch_bounds__(TX * TY, 4) void k_weno7_stage(const double *__restrict__ uin, double *uout,
+__global__ __launch_bounds__(TX * TY, 3) void k_weno7_stage(const double *__restrict__ uin, double *uout,
                                                           double *res, W7Args g, int s, double a, double b,
                                                           W7Ctrl *ctrl) {
     constexpr int NT = TX * TY, NW = NT / 64;
