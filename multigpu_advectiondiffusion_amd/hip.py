@@ -124,16 +124,32 @@ class DeviceArray:
     def size(self) -> int:
         return int(np.prod(self.shape))
 
-    def upload(self, a: np.ndarray) -> None:
+    def upload(self, a: np.ndarray, stream: Stream | None = None) -> None:
+        """Copy from the host; with a stream, ordered on that stream (and
+        finished on return), so no null-stream operation is issued."""
         a = np.ascontiguousarray(a, dtype=self.dtype)
         assert a.nbytes == self.nbytes, (a.nbytes, self.nbytes)
-        check(lib().hipMemcpy(self.ptr, a.ctypes.data_as(ctypes.c_void_p), self.nbytes, H2D), "hipMemcpy H2D")
+        src = a.ctypes.data_as(ctypes.c_void_p)
+        if stream is None:
+            check(lib().hipMemcpy(self.ptr, src, self.nbytes, H2D), "hipMemcpy H2D")
+        else:
+            check(lib().hipMemcpyAsync(self.ptr, src, self.nbytes, H2D, stream.handle), "hipMemcpyAsync H2D")
+            stream.synchronize()
 
-    def download(self) -> np.ndarray:
-        """Copy to the host after all queued device work has finished."""
-        synchronize()
+    def download(self, stream: Stream | None = None) -> np.ndarray:
+        """Copy to the host after all queued device work has finished, or,
+        with a stream, after that stream's work (a device-wide synchronisation
+        and null-stream copies are not permitted while another host thread
+        captures a graph, so callers that share the device with such threads
+        pass their stream)."""
         out = np.empty(self.shape, self.dtype)
-        check(lib().hipMemcpy(out.ctypes.data_as(ctypes.c_void_p), self.ptr, self.nbytes, D2H), "hipMemcpy D2H")
+        dst = out.ctypes.data_as(ctypes.c_void_p)
+        if stream is None:
+            synchronize()
+            check(lib().hipMemcpy(dst, self.ptr, self.nbytes, D2H), "hipMemcpy D2H")
+        else:
+            check(lib().hipMemcpyAsync(dst, self.ptr, self.nbytes, D2H, stream.handle), "hipMemcpyAsync D2H")
+            stream.synchronize()
         return out
 
     def copy_from(self, other: "DeviceArray", stream: Stream | None = None) -> None:

@@ -83,7 +83,7 @@ class PeriodicBurgers:
 
     def set_initial(self, u0: np.ndarray, t0: float = 0.0, tend: float = float("inf")) -> None:
         """Upload u0 (rounded once to float for FP32 storage) and start a run at t0."""
-        self.u.upload(u0)
+        self.u.upload(u0, self.stream)
         rc = _capi.lib().mgad_burgers_init_p(self.precision, self.stream.handle, self.nx, self.ny, self.nz,
                                              self.u.ptr, self.ctrl.ptr, t0, tend, self.cfl)
         _capi.check(rc, "mgad_burgers_init")
@@ -115,8 +115,7 @@ class PeriodicBurgers:
         return st
 
     def solution(self) -> np.ndarray:
-        self.stream.synchronize()
-        return self.u.download()
+        return self.u.download(self.stream)
 
 
 def autotune(solver: "PeriodicBurgers") -> tuple:
@@ -163,7 +162,7 @@ class SlabBurgers:
         filled on the device by the exchange in mgad_burgers_slab_init."""
         slabs = np.zeros((self.nslab, self.nzs + 6, self.ny, self.nx))
         slabs[:, 3:3 + self.nzs] = np.asarray(u0, dtype=np.float64).reshape(self.nslab, self.nzs, self.ny, self.nx)
-        self.u.upload(slabs)
+        self.u.upload(slabs, self.stream)
         rc = _capi.lib().mgad_burgers_slab_init(self.stream.handle, self.nx, self.ny, self.nzs, self.nslab,
                                                 self.u.ptr, self.ctrl.ptr, t0, tend, self.cfl)
         _capi.check(rc, "mgad_burgers_slab_init")
@@ -176,8 +175,7 @@ class SlabBurgers:
 
     def slabs(self) -> np.ndarray:
         """The whole slab set, ghosts included: (nslab, nzs + 6, ny, nx)."""
-        self.stream.synchronize()
-        return self.u.download()
+        return self.u.download(self.stream)
 
     def solution(self) -> np.ndarray:
         """The own planes gathered into (nz, ny, nx)."""
@@ -208,7 +206,7 @@ class Weno7Burgers:
         return 1.0 * np.exp(-(x[None, None, :] ** 2 + y[None, :, None] ** 2 + z[:, None, None] ** 2) / 0.1)
 
     def set_initial(self, u0: np.ndarray, t0: float = 0.0, tend: float = float("inf")) -> None:
-        self.u.upload(u0)
+        self.u.upload(u0, self.stream)
         rc = _capi.lib().mgad_weno7_init(self.stream.handle, self.nx, self.ny, self.nz, self.u.ptr, self.res.ptr,
                                          self.ctrl.ptr, t0, tend, self.cfl)
         _capi.check(rc, "mgad_weno7_init")
@@ -232,8 +230,7 @@ class Weno7Burgers:
         return t.value, dt.value, n.value
 
     def solution(self) -> np.ndarray:
-        self.stream.synchronize()
-        return self.u.download()
+        return self.u.download(self.stream)
 
 
 class BurgersBatch:
@@ -273,7 +270,7 @@ class BurgersBatch:
         (mgad_batch_init_static, no device allocation; kchunk must name the
         list, and the batch is valid until the next such call on the device)."""
         for f, u0 in zip(self.fields, u0s):
-            f["u"].upload(u0)
+            f["u"].upload(u0, self.stream)
         L = _capi.lib()
         if static_scratch:
             self.workspace = None
@@ -308,8 +305,7 @@ class BurgersBatch:
         return out
 
     def solutions(self):
-        self.stream.synchronize()
-        return [f["u"].download() for f in self.fields]
+        return [f["u"].download(self.stream) for f in self.fields]
 
 
 class BurgersAdjoint:
@@ -335,15 +331,15 @@ class BurgersAdjoint:
 
     def __call__(self, u0: np.ndarray, target: np.ndarray, tend: float, t0: float = 0.0):
         """Returns (J, dJ/du0, steps)."""
-        self.u0.upload(u0)
-        self.target.upload(target)
+        self.u0.upload(u0, self.stream)
+        self.target.upload(target, self.stream)
         J, n = ctypes.c_double(), ctypes.c_int()
         rc = _capi.lib().mgad_burgers_adjoint(self.stream.handle, ctypes.byref(self.scheme), self.nx, self.ny,
                                               self.nz, self.dx, self.dy, self.dz, self.cfl, t0, tend, self.u0.ptr,
                                               self.target.ptr, self.grad.ptr, ctypes.byref(J), self.max_steps,
                                               self.every, self.workspace.ptr, self.workspace.nbytes, ctypes.byref(n))
         _capi.check(rc, "mgad_burgers_adjoint")
-        return J.value, self.grad.download(), n.value
+        return J.value, self.grad.download(self.stream), n.value
 
 
 class ReferenceBurgers:
@@ -362,7 +358,7 @@ class ReferenceBurgers:
         self.u, self.w1, self.w2 = DeviceArray(shp), DeviceArray(shp), DeviceArray(shp)
 
     def set_initial(self, u0: np.ndarray) -> None:
-        self.u.upload(np.ascontiguousarray(u0, dtype=np.float64))
+        self.u.upload(np.ascontiguousarray(u0, dtype=np.float64), self.stream)
 
     def steps(self, nsteps: int, dt: float, graph: bool = True) -> None:
         rc = _capi.lib().mgad_burgers_ref_steps(self.stream.handle, ctypes.byref(self.scheme),
@@ -372,8 +368,7 @@ class ReferenceBurgers:
         _capi.check(rc, "mgad_burgers_ref_steps")
 
     def solution(self) -> np.ndarray:
-        self.stream.synchronize()
-        return self.u.download()
+        return self.u.download(self.stream)
 
 
 class ReferenceHeat:
@@ -389,7 +384,7 @@ class ReferenceHeat:
         self.u, self.w1, self.w2 = DeviceArray(shp), DeviceArray(shp), DeviceArray(shp)
 
     def set_initial(self, u0: np.ndarray) -> None:
-        self.u.upload(np.ascontiguousarray(u0, dtype=np.float64))
+        self.u.upload(np.ascontiguousarray(u0, dtype=np.float64), self.stream)
 
     def steps(self, nsteps: int, dt: float, graph: bool = True) -> None:
         rc = _capi.lib().mgad_heat13_rk3_steps(self.stream.handle, self.nx, self.ny, self.nz, dt, self.u.ptr,
@@ -397,8 +392,7 @@ class ReferenceHeat:
         _capi.check(rc, "mgad_heat13_rk3_steps")
 
     def solution(self) -> np.ndarray:
-        self.stream.synchronize()
-        return self.u.download()
+        return self.u.download(self.stream)
 
 
 def heat7_params(shape, C: float = 1.0, L: float = 1.0, r: float = 0.15) -> dict:
@@ -426,7 +420,7 @@ class Heat7:
         self.w = DeviceArray(shp)
 
     def set_initial(self, u0: np.ndarray) -> None:
-        self.u.upload(u0)
+        self.u.upload(u0, self.stream)
 
     def steps(self, nsteps: int, graph: bool = True) -> None:
         rc = _capi.lib().mgad_heat7_fe_steps(self.stream.handle, self.nx, self.ny, self.nz, self.kx, self.ky,
@@ -434,5 +428,4 @@ class Heat7:
         _capi.check(rc, "mgad_heat7_fe_steps")
 
     def solution(self) -> np.ndarray:
-        self.stream.synchronize()
-        return self.u.download()
+        return self.u.download(self.stream)

@@ -145,14 +145,18 @@ def test_graph_replays_match_eager(hipdev):
 def test_host_threads_share_the_library(hipdev):
     """Four host threads, each with its own stream and grid size, capture and
     replay step graphs at the same time (the graph caches and launch table are
-    shared); every result equals the same run done alone."""
+    shared); every result equals the same run done alone.  While one thread
+    captures, the others issue only work ordered on their own streams (the
+    solvers' copies are stream-ordered): a device-wide synchronisation or a
+    null-stream copy would invalidate the capture.  The solvers are created
+    before the threads start and freed after they end, since freeing device
+    memory may synchronise the device."""
     import threading
 
     sizes = (32, 48, 32, 64)
     u0s = [solvers.ic_fourier(n, seed=20 + i) for i, n in enumerate(sizes)]
 
-    def run(n, u0, out, i):
-        s = solvers.PeriodicBurgers(n, scheme="Z2")
+    def run(s, u0, out, i):
         s.set_initial(u0, tend=10.0)
         for _ in range(5):
             s.steps(4, graph=True)
@@ -160,15 +164,28 @@ def test_host_threads_share_the_library(hipdev):
 
     alone = [None] * len(sizes)
     for i, (n, u0) in enumerate(zip(sizes, u0s)):
-        run(n, u0, alone, i)
+        run(solvers.PeriodicBurgers(n, scheme="Z2"), u0, alone, i)
     together = [None] * len(sizes)
-    threads = [threading.Thread(target=run, args=(n, u0, together, i)) for i, (n, u0) in enumerate(zip(sizes, u0s))]
+    errors = []
+    objs = [solvers.PeriodicBurgers(n, scheme="Z2") for n in sizes]
+    hipdev.synchronize()
+
+    def guarded(*args):
+        try:
+            run(*args)
+        except Exception as e:  # reported below, with the case
+            errors.append(repr(e))
+
+    threads = [threading.Thread(target=guarded, args=(s, u0, together, i))
+               for i, (s, u0) in enumerate(zip(objs, u0s))]
     for t in threads:
         t.start()
     for t in threads:
         t.join(timeout=120)
+    assert not errors, errors
     for a, b in zip(alone, together):
         assert b is not None and np.array_equal(a, b)
+    del objs
 
 
 def test_no_op_after_tend(hipdev):
