@@ -39,6 +39,16 @@ struct GridArgs {
 
 __device__ __forceinline__ int wrapi(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
 
+// A value every lane of the wave holds alike (loaded from one address),
+// moved to scalar registers: it then takes no vector registers for the rest
+// of the kernel.
+__device__ __forceinline__ double wave_uniform(double x) {
+    const unsigned long long b = static_cast<unsigned long long>(__double_as_longlong(x));
+    const unsigned lo = __builtin_amdgcn_readfirstlane(static_cast<unsigned>(b));
+    const unsigned hi = __builtin_amdgcn_readfirstlane(static_cast<unsigned>(b >> 32));
+    return __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo));
+}
+
 // Step control shared by every block of a stage.  Stage 1 derives dt from
 // max|u^n| (all blocks compute the same value) and records it; stage 2 moves
 // t forward.  Each stage's lead thread clears the max|u| slot the next-but-one
@@ -73,6 +83,8 @@ __device__ __forceinline__ bool stage_control(Ctrl *ctrl, int stage, double dx, 
                 r.dt_cfl = ctrl->cfl * dx / a;
             }
         }
+        dt = wave_uniform(dt);
+        alpha = wave_uniform(alpha);
         return true;
     }
     dt = ctrl->dt;
@@ -93,6 +105,8 @@ __device__ __forceinline__ bool stage_control(Ctrl *ctrl, int stage, double dx, 
             ctrl->amax[1] = 0ull;
         }
     }
+    dt = wave_uniform(dt);
+    alpha = wave_uniform(alpha);
     return true;
 }
 
