@@ -6,17 +6,20 @@
 //    problem, so items cost about the same whatever the problem size; a grid
 //    of as many 256-thread blocks as the device keeps resident walks the list
 //    (block b takes items b, b+G, ...);
-//  - split: every item is a whole 16x16 column tile of one problem, and the
-//    items' planes, in list order, are cut into one equal share per resident
-//    block, so every block marches the same number of tile planes and no
-//    block waits on a partly filled last round of items.
-// Block ids are remapped so that the blocks of one XCD take adjacent items.
+//  - split: every item is a whole column tile of one problem, and the items'
+//    planes, in list order, are cut into shares that shrink towards the end
+//    of the list; the resident blocks draw shares from a counter until none
+//    is left, so a block that runs faster takes more of them and the blocks
+//    end together.
+// Chunked lists remap block ids so that the blocks of one XCD take adjacent
+// items.
 // Each piece of an item runs the same stage body as the single-problem
 // kernel, with the problem's own dt, LF alpha and max|u| slot (its own ctrl),
 // so results are bit-identical to advancing each problem alone.
 //
 // Workspace (device memory given by the caller):
-//   BatchProblem[nprob] | item_prob[nitems] | item_tile[nitems] | shares[G + 1] (split)
+//   BatchProblem[nprob] | item_prob[nitems] | item_tile[nitems]
+//   | ShareQueue | shares[nshares + 1]   (split lists; 8-byte aligned)
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -35,6 +38,17 @@ struct BatchProblem {
     void *u, *w1, *w2;
     Ctrl *ctrl;
     GridArgs g;
+};
+
+// Hand-out state of a split list's shares.  A launch of G blocks adds exactly
+// period = nshares + G to `next`: each block draws shares until it draws one
+// past the last, and then stops.  Launches on one stream run one after the
+// other, so `next % period` numbers the current launch's draws from 0, and
+// no reset is needed between launches or graph replays.  Written as zero with
+// the work list.
+struct ShareQueue {
+    unsigned long long next;
+    unsigned nshares, period;
 };
 
 // One piece of a work item: planes z0 .. z1-1 of the tile's columns.  The
@@ -81,25 +95,35 @@ __global__ __launch_bounds__(kBT *kBT, 4) void k_burgers_batch(const BatchProble
 }
 
 // Split list: item i is a whole column tile (all nz planes) and the items'
-// planes, taken in list order, are cut into G shares of equal plane count;
-// block b (remapped as above) runs share s from shares[s] to shares[s + 1]
-// (item, plane) pairs, shares[G] = (nitems, 0).  Every block does the same
-// number of tile planes, whatever the mix of problem sizes.  Tiles are TX x 16
-// (TX = 16 or 32) columns.
+// planes, taken in list order, are cut into q->nshares shares; share s runs
+// from shares[s] to shares[s + 1] (item, plane) pairs, shares[nshares] =
+// (nitems, 0).  Blocks draw share numbers from q (see ShareQueue) until the
+// list is used up.  The first-dispatched block on a CU issues ahead of the
+// second, so blocks march at different speeds; drawn shares let the faster
+// ones take more.  Tiles are TX x 16 (TX = 16 or 32) columns.
 template <class T, int WENO, int P, int TX>
 __global__ __launch_bounds__(TX *kBT, 4) void k_burgers_batch_split(const BatchProblem *__restrict__ probs,
                                                                     const unsigned *__restrict__ item_prob,
                                                                     const unsigned *__restrict__ item_tile,
-                                                                    const uint2 *__restrict__ shares, int nitems,
-                                                                    double eps, int stage) {
+                                                                    ShareQueue *q, const uint2 *__restrict__ shares,
+                                                                    int nitems, double eps, int stage) {
     __shared__ TileSmem<TX, kBT> sm;
-    const unsigned s = xcd_tile(blockIdx.x, gridDim.x);
-    const uint2 a = shares[s], e = shares[s + 1];
-    for (unsigned i = a.x; i <= e.x && i < static_cast<unsigned>(nitems); i++) {
-        const BatchProblem &bp = probs[item_prob[i]];
-        const int z0 = i == a.x ? static_cast<int>(a.y) : 0;
-        const int z1 = i == e.x ? static_cast<int>(e.y) : bp.g.nz;
-        if (z0 < z1) batch_piece<T, WENO, P, TX>(bp, item_tile[i], z0, z1, eps, stage, sm);
+    __shared__ unsigned drawn;
+    const unsigned nshares = q->nshares;
+    const unsigned long long period = q->period;
+    for (;;) {
+        if (threadIdx.x == 0) drawn = static_cast<unsigned>(atomicAdd(&q->next, 1ull) % period);
+        __syncthreads();
+        const unsigned s = drawn;
+        __syncthreads();  // every thread has read `drawn` before the next draw
+        if (s >= nshares) break;
+        const uint2 a = shares[s], e = shares[s + 1];
+        for (unsigned i = a.x; i <= e.x && i < static_cast<unsigned>(nitems); i++) {
+            const BatchProblem &bp = probs[item_prob[i]];
+            const int z0 = i == a.x ? static_cast<int>(a.y) : 0;
+            const int z1 = i == e.x ? static_cast<int>(e.y) : bp.g.nz;
+            if (z0 < z1) batch_piece<T, WENO, P, TX>(bp, item_tile[i], z0, z1, eps, stage, sm);
+        }
     }
 }
 
@@ -108,8 +132,8 @@ namespace {
 template <class T>
 using BatchKernel = void (*)(const BatchProblem *, const unsigned *, const unsigned *, int, double, int);
 template <class T>
-using SplitKernel = void (*)(const BatchProblem *, const unsigned *, const unsigned *, const uint2 *, int, double,
-                             int);
+using SplitKernel = void (*)(const BatchProblem *, const unsigned *, const unsigned *, ShareQueue *, const uint2 *,
+                             int, double, int);
 
 template <class T>
 BatchKernel<T> pick_batch(const mgad_scheme &sc) {
@@ -135,10 +159,17 @@ long col_tiles(const mgad_problem &p, int tw) {
     return static_cast<long>((p.nx + tw - 1) / tw) * (p.ny / kBT);
 }
 
-// descriptors | item_prob[nitems] | item_tile[nitems] | shares[nshares + 1] (split list only)
+// Byte offset of the ShareQueue of a split list (after the items, 8-aligned).
+size_t queue_offset(int nprob, long nitems) {
+    const size_t o = sizeof(BatchProblem) * nprob + 2 * sizeof(unsigned) * static_cast<size_t>(nitems);
+    return (o + 7) & ~static_cast<size_t>(7);
+}
+
+// descriptors | item_prob[nitems] | item_tile[nitems] | ShareQueue | shares[nshares + 1]
+// (the queue and shares for a split list only)
 size_t ws_bytes(int nprob, long nitems, long nshares) {
-    return sizeof(BatchProblem) * nprob + 2 * sizeof(unsigned) * static_cast<size_t>(nitems) +
-           (nshares > 0 ? sizeof(uint2) * static_cast<size_t>(nshares + 1) : 0);
+    if (nshares <= 0) return sizeof(BatchProblem) * nprob + 2 * sizeof(unsigned) * static_cast<size_t>(nitems);
+    return queue_offset(nprob, nitems) + sizeof(ShareQueue) + sizeof(uint2) * static_cast<size_t>(nshares + 1);
 }
 
 // Items of the chunked list with kchunk planes per item, or of a split list
@@ -172,38 +203,64 @@ bool problems_ok(int nprob, const mgad_problem *p) {
     return true;
 }
 
-// Cuts the planes of the split list (item k has planes[k] planes) into nshares
-// runs of equal length (to within one plane): share s starts at global plane
-// floor(s * total / nshares), written as the (item, plane) pair it falls in.
-void cut_shares(const std::vector<unsigned> &planes, long nshares, uint2 *shares) {
-    long total = 0;
-    for (unsigned n : planes) total += n;
+// Share sizes of a split list: with R tile planes not yet cut, the next share
+// takes max(kShareMin, ceil(R / (kShareDiv * grid))) of them (the last share
+// what is left).  The first round of draws covers 1/kShareDiv of the work in
+// big shares, which keeps the per-share set-up (stage control, the z queue's
+// warm-up, the block max) rare; the shares then shrink geometrically, and the
+// last ones are kShareMin planes, so the blocks end within a few planes of
+// each other.
+constexpr long kShareDiv = 2;
+constexpr long kShareMin = 8;
+
+// Global first plane of every share of a split list of `total` tile planes.
+std::vector<long> share_starts(long total, long grid) {
+    std::vector<long> st;
+    for (long pos = 0; pos < total;) {
+        st.push_back(pos);
+        const long r = total - pos;
+        pos += std::min(r, std::max(kShareMin, (r + kShareDiv * grid - 1) / (kShareDiv * grid)));
+    }
+    return st;
+}
+
+// Writes the shares (starts as global planes of the items' planes in list
+// order; item k has planes[k] planes) as the (item, plane) pairs they fall in,
+// and the end marker (nitems, 0) after them.
+void cut_shares(const std::vector<unsigned> &planes, const std::vector<long> &starts, uint2 *shares) {
     size_t k = 0;
     long before = 0;  // planes of the items ahead of item k
-    for (long s = 0; s < nshares; s++) {
-        const long start = static_cast<long>((static_cast<__int128>(s) * total) / nshares);
-        while (k < planes.size() && before + static_cast<long>(planes[k]) <= start) before += planes[k++];
-        shares[s] = make_uint2(static_cast<unsigned>(k), static_cast<unsigned>(start - before));
+    for (size_t s = 0; s < starts.size(); s++) {
+        while (k < planes.size() && before + static_cast<long>(planes[k]) <= starts[s]) before += planes[k++];
+        shares[s] = make_uint2(static_cast<unsigned>(k), static_cast<unsigned>(starts[s] - before));
     }
-    shares[nshares] = make_uint2(static_cast<unsigned>(planes.size()), 0u);
+    shares[starts.size()] = make_uint2(static_cast<unsigned>(planes.size()), 0u);
+}
+
+// Tile planes of a split list (every item a whole column of nz planes).
+long split_planes(int nprob, const mgad_problem *p, int kchunk) {
+    long work = 0;
+    for (int i = 0; i < nprob; i++) work += col_tiles(p[i], tile_w(kchunk)) * p[i].nz;
+    return work;
 }
 
 // Persistent blocks of a list's stage launch: the resident blocks, or fewer
 // when the list has less work (tile planes of a split list, items of a
 // chunked one).
 long list_grid(int nprob, const mgad_problem *p, int kchunk) {
-    const int tw = tile_w(kchunk);
-    long work = 0;
-    if (is_split(kchunk))
-        for (int i = 0; i < nprob; i++) work += col_tiles(p[i], tw) * p[i].nz;
-    else
-        work = count_items(nprob, p, kchunk);
-    return std::max(1l, std::min(work, static_cast<long>(resident_blocks(tw))));
+    const long work = is_split(kchunk) ? split_planes(nprob, p, kchunk) : count_items(nprob, p, kchunk);
+    return std::max(1l, std::min(work, static_cast<long>(resident_blocks(tile_w(kchunk)))));
 }
 
-// Workspace bytes of one list (the shares of a split list are one per block).
+// Shares of a split list (0 for a chunked one).
+long list_shares(int nprob, const mgad_problem *p, int kchunk) {
+    if (!is_split(kchunk)) return 0;
+    return static_cast<long>(share_starts(split_planes(nprob, p, kchunk), list_grid(nprob, p, kchunk)).size());
+}
+
+// Workspace bytes of one list.
 size_t list_bytes(int nprob, const mgad_problem *p, int kchunk) {
-    return ws_bytes(nprob, count_items(nprob, p, kchunk), is_split(kchunk) ? list_grid(nprob, p, kchunk) : 0);
+    return ws_bytes(nprob, count_items(nprob, p, kchunk), list_shares(nprob, p, kchunk));
 }
 
 // Writes descriptors and the work list (problems in the given order, tiles
@@ -215,8 +272,9 @@ int write_workspace(hipStream_t stream, int nprob, const mgad_problem *p, int kc
     const bool split = is_split(kchunk);
     const int tw = tile_w(kchunk);
     const long grid = list_grid(nprob, p, kchunk);
-    const long nshares = split ? grid : 0;
-    if (ws_bytes(nprob, nitems, nshares) > ws_size || nitems >= (1l << 31)) {
+    const std::vector<long> starts = split ? share_starts(split_planes(nprob, p, kchunk), grid) : std::vector<long>();
+    const long nshares = static_cast<long>(starts.size());
+    if (ws_bytes(nprob, nitems, nshares) > ws_size || nitems >= (1l << 31) || nshares >= (1l << 31) - grid) {
         set_error(MGAD_ERR_INVALID_ARG);
         return MGAD_ERR_INVALID_ARG;
     }
@@ -240,7 +298,13 @@ int write_workspace(hipStream_t stream, int nprob, const mgad_problem *p, int kc
             if (split) planes.push_back(p[i].nz);
         }
     }
-    if (split) cut_shares(planes, nshares, reinterpret_cast<uint2 *>(it + nitems));
+    if (split) {
+        ShareQueue *q = reinterpret_cast<ShareQueue *>(host.data() + queue_offset(nprob, nitems));
+        q->next = 0ull;
+        q->nshares = static_cast<unsigned>(nshares);
+        q->period = static_cast<unsigned>(nshares + grid);
+        cut_shares(planes, starts, reinterpret_cast<uint2 *>(q + 1));
+    }
     int st = check_hip(hipMemcpyAsync(ws, host.data(), host.size(), hipMemcpyHostToDevice, stream));
     if (!st) st = check_hip(hipStreamSynchronize(stream));
     b.nitems = static_cast<int>(nitems);
@@ -261,8 +325,10 @@ int launch_batch_stage(hipStream_t stream, const mgad_batch &b, int stage) {
             set_error(MGAD_ERR_UNSUPPORTED);
             return MGAD_ERR_UNSUPPORTED;
         }
-        hipLaunchKernelGGL(fn, dim3(b.grid), dim3(tw * kBT), 0, stream, probs, ip, it,
-                           reinterpret_cast<const uint2 *>(it + b.nitems), b.nitems, b.scheme.eps, stage);
+        ShareQueue *q = reinterpret_cast<ShareQueue *>(static_cast<unsigned char *>(b.workspace) +
+                                                       queue_offset(b.nprob, b.nitems));
+        hipLaunchKernelGGL(fn, dim3(b.grid), dim3(tw * kBT), 0, stream, probs, ip, it, q,
+                           reinterpret_cast<const uint2 *>(q + 1), b.nitems, b.scheme.eps, stage);
     } else {
         BatchKernel<T> fn = pick_batch<T>(b.scheme);
         if (!fn) {
@@ -399,8 +465,10 @@ using namespace mgad;
 extern "C" size_t mgad_batch_workspace_bytes(int nprob, const mgad_problem *probs) {
     if (nprob <= 0 || !probs) return 0;
     // the chunked list of the smallest chunk the tuner tries holds at least
-    // as many items as the split list, which adds one share per block
-    return ws_bytes(nprob, count_items(nprob, probs, 4), resident_blocks(kBT));
+    // as many items as the other chunked lists; the split lists add shares
+    size_t n = list_bytes(nprob, probs, 4);
+    for (int kc : {MGAD_BATCH_SPLIT, MGAD_BATCH_SPLIT_WIDE}) n = std::max(n, list_bytes(nprob, probs, kc));
+    return n;
 }
 
 namespace {
