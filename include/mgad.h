@@ -347,18 +347,26 @@ int mgad_burgers_steps_p(int precision, hipStream_t stream,
                          double dy, double dz, void *u, void *w1, void *w2,
                          void *ctrl, int nsteps, int use_graph);
 
+/* kchunk of a stage launch whose z bands shrink towards the top of the grid
+ * (big ones first, 4- or 8-plane ones last). */
+#define MGAD_BANDS_SHRINK (-1)
+
 /* Autotuner: times stage-1 launches (u -> w1) of every candidate tile width
- * (16, 32) and z-chunk depth (4..128 planes, nz) on the caller's buffers and
- * records the fastest for this (precision, scheme, grid); later launches of
- * that grid use it.  Overwrites w1 and ctrl: call mgad_burgers_init[_p]
- * afterwards.  Results do not depend on the configuration (bit-identical). */
+ * (16, 32) and z band rule on the caller's buffers and records the fastest
+ * for this (precision, scheme, grid); later launches of that grid use it.  A
+ * stage launch runs one workgroup per column tile and band of z planes;
+ * kchunk > 0 makes the bands kchunk planes deep (4..128 and nz are tried),
+ * MGAD_BANDS_SHRINK makes them shrink towards the top of the grid.
+ * Overwrites w1 and ctrl: call mgad_burgers_init[_p] afterwards.  Results do
+ * not depend on the configuration (bit-identical). */
 int mgad_burgers_autotune(hipStream_t stream, int precision,
                           const mgad_scheme *scheme, unsigned int nx,
                           unsigned int ny, unsigned int nz, double dx,
                           double dy, double dz, const void *u, void *w1,
                           void *ctrl, int *tile, int *kchunk);
 /* Sets the launch configuration of a grid explicitly (tile 0 / kchunk 0 =
- * the built-in default). */
+ * the built-in default; kchunk > 0 = bands of that depth, MGAD_BANDS_SHRINK
+ * = shrinking bands). */
 int mgad_burgers_set_launch(int precision, const mgad_scheme *scheme,
                             unsigned int nx, unsigned int ny, unsigned int nz,
                             int tile, int kchunk);

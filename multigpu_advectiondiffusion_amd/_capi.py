@@ -19,6 +19,7 @@ LF_LOCAL, LF_GLOBAL = 0, 1
 CTRL_BYTES = 256
 F64, F32 = 0, 1
 BATCH_SPLIT, BATCH_SPLIT_WIDE = -1, -2  # kchunk of the split batch work lists (16x16 / 32x16 tiles)
+BANDS_SHRINK = -1  # kchunk of a single-problem stage launch with z bands that shrink towards the top
 
 # Every symbol include/mgad.h declares.
 EXPORTS = (

@@ -108,14 +108,11 @@ __global__ __launch_bounds__(TX *kBT, 4) void k_burgers_batch_split(const BatchP
                                                                     ShareQueue *q, const uint2 *__restrict__ shares,
                                                                     int nitems, double eps, int stage) {
     __shared__ TileSmem<TX, kBT> sm;
-    __shared__ unsigned drawn;
+    __shared__ unsigned long long drawn;
     const unsigned nshares = q->nshares;
     const unsigned long long period = q->period;
     for (;;) {
-        if (threadIdx.x == 0) drawn = static_cast<unsigned>(atomicAdd(&q->next, 1ull) % period);
-        __syncthreads();
-        const unsigned s = drawn;
-        __syncthreads();  // every thread has read `drawn` before the next draw
+        const unsigned s = static_cast<unsigned>(block_draw(&q->next, drawn) % period);
         if (s >= nshares) break;
         const uint2 a = shares[s], e = shares[s + 1];
         for (unsigned i = a.x; i <= e.x && i < static_cast<unsigned>(nitems); i++) {

@@ -8,11 +8,16 @@
 // Compute_RK (kernels.cu:520-527) and max|u_out| for the next stage's LF
 // alpha, so u is read once and written once per stage (16-24 B per node).
 //
-// Work decomposition: a workgroup owns a TX x TY column tile and a chunk of
-// z planes.  Each thread owns one (x,y) column and marches up the chunk with
-// a 7-plane register queue of split fluxes (and of the 13/12 smoothness
-// terms), so every z face is computed once and reused by the two nodes that
-// share it.  For x and y faces the current plane's split fluxes (tile + 3-node
+// Work decomposition: a workgroup owns a TX x TY column tile over a band of z
+// planes; workgroups are numbered band by band.  Bands are kchunk planes deep
+// (by default from default_kchunk), or, with kchunk = MGAD_BANDS_SHRINK, they
+// shrink towards the top of the grid: the dispatcher hands out workgroups in
+// order as resident ones finish, so the launch then ends with short ones (a
+// CU's first-dispatched workgroup issues ahead of its second, so workgroups
+// end at uneven times).  Each thread owns one (x,y) column and marches up the
+// band with a 7-plane register queue of split fluxes (and of the 13/12
+// smoothness terms), so every z face is computed once and reused by the two
+// nodes that share it.  For x and y faces the current plane's split fluxes (tile + 3-node
 // halo) are staged in LDS; each thread computes the face on its +1/2 side and
 // one wave, rotating with the plane index, computes the TY + TX faces on the
 // tile's -1/2 edges, so every face of the plane is computed once.
@@ -36,21 +41,30 @@ namespace mgad {
 
 // XW: nx or ny is not a multiple of the tile; the last tile of each row or
 // column reaches past the grid and stores only its real nodes.
+// Workgroup w covers band w / ntiles; within a band, the tiles (x-fastest,
+// then y) are remapped so that each XCD's workgroups (w mod 8 under
+// round-robin dispatch) take a contiguous run of them and share their halo
+// rows in that XCD's L2.
 template <class T, int WENO, int P, int TX, int TY, bool XW>
 __global__ __launch_bounds__(TX * TY, 4) void k_burgers_stage(const T *__restrict__ uin, const T *un,
-                                                           T *uout, GridArgs g, double eps, int stage,
-                                                           Ctrl *ctrl) {
+                                                           T *uout, GridArgs g, ZBands bands, double eps,
+                                                           int stage, Ctrl *ctrl) {
     __shared__ TileSmem<TX, TY> sm;
     double dt, alpha;
     const bool lead = (blockIdx.x == 0 && threadIdx.x == 0);
     if (!stage_control(ctrl, stage, g.dx, lead, dt, alpha)) return;
-    const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
+    const unsigned ntx = XW ? (g.nx + TX - 1) / TX : g.nx / TX;
+    const unsigned ntiles = ntx * (XW ? (g.ny + TY - 1) / TY : g.ny / TY);
+    const unsigned b = blockIdx.x / ntiles;
+    const unsigned t = xcd_tile(blockIdx.x - b * ntiles, ntiles);
+    const int x0 = static_cast<int>(t % ntx) * TX, y0 = static_cast<int>(t / ntx) * TY;
+    const int z0 = bands.z[b], z1 = bands.z[b + 1];
     if (fast_division_ok<WENO>(alpha, eps))
-        stage_tile<T, WENO, P, TX, TY, true, false, XW>(uin, un, uout, g, stage, tile, dt, alpha, eps, sm,
-                                                        &ctrl->amax[stage % 3]);
+        stage_cols<T, WENO, P, TX, TY, true, false, XW>(uin, un, uout, g, stage, x0, y0, z0, z1, dt, alpha, eps, sm,
+                                                        &ctrl->amax[stage % 3], nullptr, nullptr);
     else
-        stage_tile<T, WENO, P, TX, TY, false, false, XW>(uin, un, uout, g, stage, tile, dt, alpha, eps, sm,
-                                                         &ctrl->amax[stage % 3]);
+        stage_cols<T, WENO, P, TX, TY, false, false, XW>(uin, un, uout, g, stage, x0, y0, z0, z1, dt, alpha, eps,
+                                                         sm, &ctrl->amax[stage % 3], nullptr, nullptr);
 }
 
 // max|u| over n values into *slot (atomicMax of the bit pattern).
@@ -92,13 +106,23 @@ __global__ void k_ctrl_init(Ctrl *c, double t0, double tend, double cfl) {
 namespace {
 
 template <class T>
-using StageKernel = void (*)(const T *, const T *, T *, GridArgs, double, int, Ctrl *);
+using StageKernel = void (*)(const T *, const T *, T *, GridArgs, ZBands, double, int, Ctrl *);
 
 template <class T>
 struct Shape {
     int tx, ty;
     StageKernel<T> fn;
 };
+
+// Workgroups of `threads` threads the device keeps resident: the stage
+// kernels' launch bounds hold them to 4 waves per SIMD, 16 waves per CU.
+int resident_groups(int threads) {
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+        cus = 256;
+    return cus * 16 / (threads / 64);
+}
 
 // Tiles that divide the grid when they can: 32x16 (unless 16 is asked for),
 // else 16x16; otherwise tiles that reach past the grid, 32x16 for rows of 64
@@ -129,6 +153,33 @@ bool pick_kernel(const mgad_scheme &sc, int nx, int ny, int tile_pref, Shape<T> 
 }
 
 }  // namespace
+
+ZBands make_bands(int nz, long ntiles, long groups, int kchunk) {
+    ZBands b{};
+    if (kchunk > 0) {
+        const int len = std::max(kchunk, (nz + kMaxBands - 1) / kMaxBands);
+        for (int z = 0; z < nz; z += len) b.z[b.n++] = z;
+        b.z[b.n] = nz;
+        return b;
+    }
+    // Shrinking bands: with R planes left, the next band has
+    // ceil(R * ntiles / (2 * max(groups, ntiles))) planes, at least lmin, so
+    // the first round of resident workgroups covers about half the work and
+    // the sizes then fall geometrically.  lmin is 8 planes (the 6-plane queue
+    // warm-up of a workgroup stays small), or 4 when the grid gives the
+    // resident workgroups fewer than 32 tile planes each.
+    const long den = 2 * std::max(groups, ntiles);
+    const long lmin = ntiles * nz >= 32 * groups ? 8 : 4;
+    for (long pos = 0; pos < nz;) {
+        b.z[b.n++] = static_cast<int>(pos);
+        const long r = nz - pos;
+        long len = std::max(lmin, (r * ntiles + den - 1) / den);
+        if (b.n == kMaxBands) len = r;
+        pos += std::min(len, r);
+    }
+    b.z[b.n] = nz;
+    return b;
+}
 
 int default_kchunk(int nx, int ny, int nz, int tx, int ty) {
     // Enough blocks to give every CU several tiles; chunks of >= 8 planes so
@@ -177,11 +228,17 @@ int burgers_stage_launch_t(hipStream_t stream, const mgad_scheme &sc, unsigned n
         set_error(MGAD_ERR_UNSUPPORTED);
         return MGAD_ERR_UNSUPPORTED;
     }
-    if (kchunk <= 0) kchunk = default_kchunk(nx, ny, nz, s.tx, s.ty);
-    const unsigned nblocks = ((nx + s.tx - 1) / s.tx) * ((ny + s.ty - 1) / s.ty) * ((nz + kchunk - 1) / kchunk);
+    const long ntiles = static_cast<long>((nx + s.tx - 1) / s.tx) * ((ny + s.ty - 1) / s.ty);
+    if (kchunk == 0) kchunk = default_kchunk(nx, ny, nz, s.tx, s.ty);
+    const ZBands bands = make_bands(static_cast<int>(nz), ntiles, resident_groups(s.tx * s.ty), kchunk);
+    const long groups = ntiles * bands.n;
+    if (groups >= (1l << 31)) {
+        set_error(MGAD_ERR_UNSUPPORTED);
+        return MGAD_ERR_UNSUPPORTED;
+    }
     const GridArgs g = make_grid(nx, ny, nz, dx, dy, dz, kchunk);
-    hipLaunchKernelGGL(s.fn, dim3(nblocks), dim3(s.tx * s.ty), 0, stream, u_in, u_n, u_out, g, sc.eps,
-                       stage, static_cast<Ctrl *>(ctrl));
+    hipLaunchKernelGGL(s.fn, dim3(static_cast<unsigned>(groups)), dim3(s.tx * s.ty), 0, stream, u_in, u_n, u_out,
+                       g, bands, sc.eps, stage, static_cast<Ctrl *>(ctrl));
     return check_launch();
 }
 
@@ -256,8 +313,9 @@ void set_launch_cfg(int precision, const mgad_scheme &sc, unsigned nx, unsigned 
     cfg_table()[cfg_key(precision, sc, nx, ny, nz)] = c;
 }
 
-// Times stage-1 launches (u -> w1) for each candidate tile width and z chunk
-// depth with HIP events on the caller's stream; keeps the fastest.
+// Times stage-1 launches (u -> w1) for each candidate tile width and z band
+// rule (shrinking bands, bands of 4..128 planes and of nz) with HIP events on
+// the caller's stream; keeps the fastest.
 template <class T>
 int autotune_t(hipStream_t stream, const mgad_scheme &sc, unsigned nx, unsigned ny, unsigned nz, double dx,
                double dy, double dz, const T *u, T *w1, void *ctrl, LaunchCfg *best_out) {
@@ -268,7 +326,7 @@ int autotune_t(hipStream_t stream, const mgad_scheme &sc, unsigned nx, unsigned 
     std::vector<int> tiles;
     if (nx % 32 == 0 || (nx % 16 != 0 && nx >= 32) || ny % 16 != 0) tiles.push_back(32);
     tiles.push_back(16);
-    std::vector<int> chunks;
+    std::vector<int> chunks{MGAD_BANDS_SHRINK};
     for (int kc : {4, 8, 16, 32, 64, 128})
         if (kc <= static_cast<int>(nz)) chunks.push_back(kc);
     chunks.push_back(static_cast<int>(nz));
@@ -495,7 +553,8 @@ extern "C" int mgad_burgers_autotune(hipStream_t stream, int precision, const mg
 
 extern "C" int mgad_burgers_set_launch(int precision, const mgad_scheme *scheme, unsigned int nx,
                                        unsigned int ny, unsigned int nz, int tile, int kchunk) {
-    if (!scheme || !precision_ok(precision) || (tile != 0 && tile != 16 && tile != 32) || kchunk < 0) {
+    if (!scheme || !precision_ok(precision) || (tile != 0 && tile != 16 && tile != 32) ||
+        (kchunk < 0 && kchunk != MGAD_BANDS_SHRINK)) {
         set_error(MGAD_ERR_INVALID_ARG);
         return MGAD_ERR_INVALID_ARG;
     }

@@ -358,6 +358,17 @@ __device__ __forceinline__ void stage_tile(const T *__restrict__ uin, const T *u
 
 int default_kchunk(int nx, int ny, int nz, int tx, int ty);
 
+// z bands of a stage launch: band b covers planes z[b] .. z[b+1]-1.
+constexpr int kMaxBands = 128;
+struct ZBands {
+    int n;
+    int z[kMaxBands + 1];
+};
+// Bands of kchunk > 0 planes (more when nz / kchunk exceeds kMaxBands), or for
+// kchunk = MGAD_BANDS_SHRINK bands that shrink towards the top, sized from the
+// tiles per plane and the workgroups the device keeps resident.
+ZBands make_bands(int nz, long ntiles, long groups, int kchunk);
+
 // Tile width (16 or 32; 0 = widest that divides nx) and planes per block
 // (0 = default_kchunk) of the single-problem stage launch.
 struct LaunchCfg {

@@ -317,4 +317,40 @@ __device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v)
     return v;
 }
 
+// Draws the next work-share number for the whole block from *counter (one
+// added per draw) and returns it, wave-uniform, to every thread.  Wave 0 draws
+// with all its lanes active (lane 0 adds 1, the others 0, and lane 0's old
+// value is the draw) inside a branch on a wave-uniform value, so no lane runs
+// apart from its wave around the two barriers: with the draw under
+// `threadIdx.x == 0` the compiler may merge it with another lane-0 region of
+// the loop and move it out of the barrier loop, which then never ends.
+// slot: a __shared__ word of the caller.
+template <class C>
+__device__ __forceinline__ C block_draw(C *counter, C &slot) {
+    const unsigned wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (wave == 0u) {
+        const C v = atomicAdd(counter, (threadIdx.x & 63u) == 0u ? C(1) : C(0));
+        if (sizeof(C) == 8) {
+            const unsigned long long b = static_cast<unsigned long long>(v);
+            const unsigned lo = __builtin_amdgcn_readfirstlane(static_cast<unsigned>(b));
+            const unsigned hi = __builtin_amdgcn_readfirstlane(static_cast<unsigned>(b >> 32));
+            slot = static_cast<C>((static_cast<unsigned long long>(hi) << 32) | lo);
+        } else {
+            slot = static_cast<C>(__builtin_amdgcn_readfirstlane(static_cast<unsigned>(v)));
+        }
+    }
+    __syncthreads();
+    C s;
+    if (sizeof(C) == 8) {
+        const unsigned long long b = static_cast<unsigned long long>(slot);
+        const unsigned lo = __builtin_amdgcn_readfirstlane(static_cast<unsigned>(b));
+        const unsigned hi = __builtin_amdgcn_readfirstlane(static_cast<unsigned>(b >> 32));
+        s = static_cast<C>((static_cast<unsigned long long>(hi) << 32) | lo);
+    } else {
+        s = static_cast<C>(__builtin_amdgcn_readfirstlane(static_cast<unsigned>(slot)));
+    }
+    __syncthreads();  // every thread has read the slot before the next draw
+    return s;
+}
+
 }  // namespace mgad

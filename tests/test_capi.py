@@ -53,6 +53,12 @@ def test_invalid_arguments_are_reported_without_device():
     assert _capi.last_error() == _capi.MGAD_ERR_INVALID_ARG
     assert L.mgad_set_adv_scheme(5, 1, 1e-6, 0, 0.0) == _capi.MGAD_ERR_INVALID_ARG
     assert _capi.last_error() == _capi.MGAD_ERR_INVALID_ARG
+    # launch rules: bands of kchunk > 0 planes, 0 (default) or shrinking bands; nothing else
+    assert L.mgad_burgers_set_launch(_capi.F64, ctypes.byref(sc), 40, 40, 40, 32, _capi.BANDS_SHRINK) == 0
+    assert L.mgad_burgers_set_launch(_capi.F64, ctypes.byref(sc), 40, 40, 40, 32, -2) == _capi.MGAD_ERR_INVALID_ARG
+    assert L.mgad_burgers_set_launch(_capi.F64, ctypes.byref(sc), 40, 40, 40, 8, 0) == _capi.MGAD_ERR_INVALID_ARG
+    assert L.mgad_burgers_set_launch(_capi.F64, ctypes.byref(sc), 40, 40, 40, 0, 0) == 0
+    _capi.last_error()
     assert _capi.error_string(_capi.MGAD_ERR_UNSUPPORTED) == "unsupported grid shape"
 
 
