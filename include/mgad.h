@@ -351,14 +351,17 @@ int mgad_burgers_steps_p(int precision, hipStream_t stream,
  * (big ones first, 4- or 8-plane ones last). */
 #define MGAD_BANDS_SHRINK (-1)
 
-/* Autotuner: times stage-1 launches (u -> w1) of every candidate tile width
- * (16, 32) and z band rule on the caller's buffers and records the fastest
- * for this (precision, scheme, grid); later launches of that grid use it.  A
- * stage launch runs one workgroup per column tile and band of z planes;
- * kchunk > 0 makes the bands kchunk planes deep (4..128 and nz are tried),
- * MGAD_BANDS_SHRINK makes them shrink towards the top of the grid.
- * Overwrites w1 and ctrl: call mgad_burgers_init[_p] afterwards.  Results do
- * not depend on the configuration (bit-identical). */
+/* Autotuner: times stage-1 launches (u -> w1) of the default launch and of
+ * every candidate tile width (16, 32) and z band rule on the caller's buffers
+ * and records the fastest for this (precision, scheme, grid); later launches
+ * of that grid use it.  A stage launch runs one workgroup per column tile and
+ * band of z planes.  The default launch (tile 0, kchunk 0) uses 32x32 tiles,
+ * one workgroup per CU, when 32 divides nx and ny and one round of them
+ * covers the grid (bands of >= 4 planes filling >= 90 % of the CUs), and
+ * 32x16 tiles otherwise; kchunk > 0 makes the bands kchunk planes deep
+ * (4..128 and nz are tried), MGAD_BANDS_SHRINK makes them shrink towards the
+ * top of the grid.  Overwrites w1 and ctrl: call mgad_burgers_init[_p]
+ * afterwards.  Results do not depend on the configuration (bit-identical). */
 int mgad_burgers_autotune(hipStream_t stream, int precision,
                           const mgad_scheme *scheme, unsigned int nx,
                           unsigned int ny, unsigned int nz, double dx,

@@ -124,11 +124,17 @@ int resident_groups(int threads) {
     return cus * 16 / (threads / 64);
 }
 
-// Tiles that divide the grid when they can: 32x16 (unless 16 is asked for),
-// else 16x16; otherwise tiles that reach past the grid, 32x16 for rows of 64
-// nodes or more (or when 32 is asked for), else 16x16.
+// Tiles that divide the grid when they can: 32x32 when asked for (kSquare),
+// 32x16 (unless 16 is asked for), else 16x16; otherwise tiles that reach past
+// the grid, 32x16 for rows of 64 nodes or more (or when 32 is asked for),
+// else 16x16.
+constexpr int kSquare = 1024;
 template <class T, int WENO, int P>
 bool pick(int nx, int ny, int tile_pref, Shape<T> &s) {
+    if (tile_pref == kSquare && nx % 32 == 0 && ny % 32 == 0) {
+        s = {32, 32, k_burgers_stage<T, WENO, P, 32, 32, false>};
+        return true;
+    }
     if (tile_pref != 16 && nx % 32 == 0 && ny % 16 == 0) {
         s = {32, 16, k_burgers_stage<T, WENO, P, 32, 16, false>};
         return true;
@@ -181,10 +187,24 @@ ZBands make_bands(int nz, long ntiles, long groups, int kchunk) {
     return b;
 }
 
+// The default launch's 32x32 bands: a 16-wave workgroup of 32x32 columns
+// fills a CU, so one round of one workgroup per CU over the whole grid has no
+// CU holding two workgroups (the first-dispatched of two issues ahead of the
+// second, so pairs end at uneven times) and no CU waiting for a next one.  It
+// is used when 32 divides nx and ny and (tiles per plane) x (bands of at least
+// 4 planes) fills 90 % of the CUs in one round; returns the band count, or 0.
+int square_bands(unsigned nx, unsigned ny, unsigned nz) {
+    if (nx % 32 != 0 || ny % 32 != 0) return 0;
+    const long tiles = static_cast<long>(nx / 32) * (ny / 32);
+    const long cus = resident_groups(1024);
+    const long b = std::min<long>({cus / tiles, static_cast<long>(nz) / 4, kMaxBands});
+    return b >= 1 && 10 * tiles * b >= 9 * cus ? static_cast<int>(b) : 0;
+}
+
 int default_kchunk(int nx, int ny, int nz, int tx, int ty) {
+    const long tiles = static_cast<long>((nx + tx - 1) / tx) * ((ny + ty - 1) / ty);
     // Enough blocks to give every CU several tiles; chunks of >= 8 planes so
     // the 6-plane queue warm-up stays small.
-    const long tiles = static_cast<long>((nx + tx - 1) / tx) * ((ny + ty - 1) / ty);
     int kc = nz;
     while (kc > 8 && tiles * ((nz + kc - 1) / kc) < 2048) kc = (kc + 1) / 2;
     return kc;
@@ -222,6 +242,13 @@ int burgers_stage_launch_t(hipStream_t stream, const mgad_scheme &sc, unsigned n
         u_out == u_in || !(dx > 0) || !(dy > 0) || !(dz > 0)) {
         set_error(MGAD_ERR_INVALID_ARG);
         return MGAD_ERR_INVALID_ARG;
+    }
+    if (tile_pref == 0 && kchunk == 0) {
+        const int b = square_bands(nx, ny, nz);
+        if (b > 0) {
+            tile_pref = kSquare;
+            kchunk = static_cast<int>((nz + b - 1) / b);
+        }
     }
     Shape<T> s;
     if (!pick_kernel<T>(sc, static_cast<int>(nx), static_cast<int>(ny), tile_pref, s)) {
@@ -326,29 +353,33 @@ int autotune_t(hipStream_t stream, const mgad_scheme &sc, unsigned nx, unsigned 
     std::vector<int> tiles;
     if (nx % 32 == 0 || (nx % 16 != 0 && nx >= 32) || ny % 16 != 0) tiles.push_back(32);
     tiles.push_back(16);
-    std::vector<int> chunks{MGAD_BANDS_SHRINK};
+    std::vector<int> chunks{0, MGAD_BANDS_SHRINK};  // 0: default_kchunk
     for (int kc : {4, 8, 16, 32, 64, 128})
         if (kc <= static_cast<int>(nz)) chunks.push_back(kc);
     chunks.push_back(static_cast<int>(nz));
     LaunchCfg best{0, 0};
     float best_ms = 1e30f;
     const int reps = 3;
+    // the default launch (one round of 32x32 workgroups where it fits) first
+    std::vector<LaunchCfg> cands{LaunchCfg{0, 0}};
     for (int t : tiles)
-        for (int kc : chunks) {
+        for (int kc : chunks) cands.push_back(LaunchCfg{t, kc});
+    for (const LaunchCfg &cand : cands) {
+        const int t = cand.tile, kc = cand.kchunk;
+        st = burgers_stage_launch_t<T>(stream, sc, nx, ny, nz, dx, dy, dz, 1, u, u, w1, ctrl, kc, t);
+        if (!st) st = check_hip(hipEventRecord(e0, stream));
+        for (int r = 0; r < reps && !st; r++)
             st = burgers_stage_launch_t<T>(stream, sc, nx, ny, nz, dx, dy, dz, 1, u, u, w1, ctrl, kc, t);
-            if (!st) st = check_hip(hipEventRecord(e0, stream));
-            for (int r = 0; r < reps && !st; r++)
-                st = burgers_stage_launch_t<T>(stream, sc, nx, ny, nz, dx, dy, dz, 1, u, u, w1, ctrl, kc, t);
-            if (!st) st = check_hip(hipEventRecord(e1, stream));
-            if (!st) st = check_hip(hipEventSynchronize(e1));
-            float ms = 0.f;
-            if (!st) st = check_hip(hipEventElapsedTime(&ms, e0, e1));
-            if (st) break;
-            if (ms < best_ms) {
-                best_ms = ms;
-                best = LaunchCfg{t, kc};
-            }
+        if (!st) st = check_hip(hipEventRecord(e1, stream));
+        if (!st) st = check_hip(hipEventSynchronize(e1));
+        float ms = 0.f;
+        if (!st) st = check_hip(hipEventElapsedTime(&ms, e0, e1));
+        if (st) break;
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = cand;
         }
+    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (st) return st;
