@@ -187,18 +187,18 @@ ZBands make_bands(int nz, long ntiles, long groups, int kchunk) {
     return b;
 }
 
-// The default launch's 32x32 bands: a 16-wave workgroup of 32x32 columns
-// fills a CU, so one round of one workgroup per CU over the whole grid has no
-// CU holding two workgroups (the first-dispatched of two issues ahead of the
-// second, so pairs end at uneven times) and no CU waiting for a next one.  It
-// is used when 32 divides nx and ny and (tiles per plane) x (bands of at least
-// 4 planes) fills 90 % of the CUs in one round; returns the band count, or 0.
-int square_bands(unsigned nx, unsigned ny, unsigned nz) {
-    if (nx % 32 != 0 || ny % 32 != 0) return 0;
-    const long tiles = static_cast<long>(nx / 32) * (ny / 32);
+int square_bands(long tiles, unsigned nz) {
     const long cus = resident_groups(1024);
+    if (tiles < 1) return 0;
     const long b = std::min<long>({cus / tiles, static_cast<long>(nz) / 4, kMaxBands});
     return b >= 1 && 10 * tiles * b >= 9 * cus ? static_cast<int>(b) : 0;
+}
+
+// square_bands of a grid whose tiles are 32x32 columns, or 0 when 32 does
+// not divide nx and ny.
+int square_bands(unsigned nx, unsigned ny, unsigned nz) {
+    if (nx % 32 != 0 || ny % 32 != 0) return 0;
+    return square_bands(static_cast<long>(nx / 32) * (ny / 32), nz);
 }
 
 int default_kchunk(int nx, int ny, int nz, int tx, int ty) {

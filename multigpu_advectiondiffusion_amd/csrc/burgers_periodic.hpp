@@ -358,6 +358,15 @@ __device__ __forceinline__ void stage_tile(const T *__restrict__ uin, const T *u
 
 int default_kchunk(int nx, int ny, int nz, int tx, int ty);
 
+// Bands of the default launch's 32x32 tiles.  A 16-wave workgroup of 32x32
+// columns fills a CU, so one round of one workgroup per CU over the whole grid
+// has no CU holding two workgroups (the first-dispatched of two issues ahead
+// of the second, so pairs end at uneven times) and no CU waiting for a next
+// one.  For `tiles` 32x32 column tiles over nz planes: the band count b (bands
+// of at least 4 planes) with which tiles x b workgroups fill at least 90 % of
+// the CUs in one round, or 0 when no such b exists.
+int square_bands(long tiles, unsigned nz);
+
 // z bands of a stage launch: band b covers planes z[b] .. z[b+1]-1.
 constexpr int kMaxBands = 128;
 struct ZBands {

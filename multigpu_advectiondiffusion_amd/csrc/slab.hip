@@ -104,8 +104,17 @@ namespace {
 
 using SlabKernel = void (*)(const double *, const double *, double *, GridArgs, unsigned, double, int, Ctrl *);
 
+// tile_pref kSlabSquare: 32x32 tiles (one 16-wave workgroup per CU).
+constexpr int kSlabSquare = 1024;
+
 template <int WENO, int P>
 bool pick_slab(unsigned nx, unsigned ny, int tile_pref, SlabKernel &fn, int &tx, int &ty) {
+    if (tile_pref == kSlabSquare && nx % 32 == 0 && ny % 32 == 0) {
+        fn = k_burgers_slab_stage<double, WENO, P, 32, 32>;
+        tx = 32;
+        ty = 32;
+        return true;
+    }
     if (tile_pref != 16 && nx % 32 == 0 && ny % 16 == 0) {
         fn = k_burgers_slab_stage<double, WENO, P, 32, 16>;
         tx = 32;
@@ -148,7 +157,13 @@ int enqueue_slab_step(hipStream_t s, const mgad_scheme &sc, unsigned nx, unsigne
     // the launch table of the unsplit grid of the same size (environment
     // overrides, mgad_burgers_set_launch / autotune entries), chunks capped at
     // one slab
-    const LaunchCfg cfg = launch_cfg(MGAD_F64, sc, nx, ny, nzs * nslab);
+    LaunchCfg cfg = launch_cfg(MGAD_F64, sc, nx, ny, nzs * nslab);
+    // as on the unsplit grid, the default launch is one round of 32x32
+    // workgroups, one per CU, where the slabs' tiles fill the CUs
+    if (cfg.tile == 0 && cfg.kchunk == 0 && nx % 32 == 0 && ny % 32 == 0) {
+        const int b = square_bands(static_cast<long>(nx / 32) * (ny / 32) * nslab, nzs);
+        if (b > 0) cfg = LaunchCfg{kSlabSquare, static_cast<int>((nzs + b - 1) / b)};
+    }
     SlabKernel fn = nullptr;
     int tx = 0, ty = 0;
     if (!pick_slab_kernel(sc, nx, ny, cfg.tile, fn, tx, ty)) {

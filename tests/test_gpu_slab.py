@@ -34,6 +34,23 @@ def test_slabs_match_unsplit_run(hipdev, nslab, scheme, graph):
     assert np.array_equal(got, p.solution())
 
 
+@pytest.mark.parametrize("nslab,nzs", [(1, 16), (2, 16), (4, 8), (8, 4)])
+def test_slabs_full_plane_match_unsplit_run(hipdev, nslab, nzs):
+    """256 x 256 planes: 1, 2 and 4 slabs take the default one-round launch of
+    32x32 workgroups (one per CU); 8 slabs of 4 planes have too many tiles
+    for one round and take 32x16 tiles.  Bit for bit against the unsplit
+    grid."""
+    shape = (256, 256, nslab * nzs)
+    u0 = solvers.ic_fourier(shape, seed=12)
+    s = solvers.SlabBurgers(shape, nslab, scheme="Z2", cfl=0.4)
+    s.set_initial(u0, tend=10.0)
+    s.steps(3, graph=True)
+    p = solvers.PeriodicBurgers(shape, scheme="Z2", cfl=0.4)
+    p.set_initial(u0, tend=10.0)
+    p.steps(3, graph=True)
+    assert np.array_equal(s.solution(), p.solution())
+
+
 def test_slabs_uneven_shape_and_t_end(hipdev):
     """48 x 16 x 30 in 5 slabs of 6 planes, 16-wide tiles, run past t_end: the
     clipped last step and the no-op steps after it as on the unsplit grid."""
