@@ -400,7 +400,8 @@ typedef struct {
 } mgad_batch;
 
 /* kchunk of the split work lists: one item per column tile (all its planes),
- * the items' planes cut into one equal share per resident block.  Tiles are
+ * the items' planes cut into shares that shrink towards the end of the list,
+ * which the resident blocks draw from a counter in the workspace.  Tiles are
  * 16x16 columns, or 32x16 for the wide list (where nx % 32 == 16, the last
  * tile of a row reaches past nx and stores only its first 16 columns). */
 #define MGAD_BATCH_SPLIT      (-1)
