@@ -286,9 +286,15 @@ RefLaunch ref_launch(const mgad_scheme &sc, bool visc, unsigned nx, unsigned ny,
                      double dz) {
     RefLaunch L{};
     const int ix = static_cast<int>(nx) - 6, iy = static_cast<int>(ny) - 6, iz = static_cast<int>(nz) - 6;
+    // one round of 32x32 workgroups, one per CU, when the interior's tiles
+    // fill the CUs (as for the periodic stage); else 32x16 or 16x16 tiles
+    const int sq = ix > 0 && iy > 0 && iz > 0
+                       ? square_bands(static_cast<long>((ix + 31) / 32) * ((iy + 31) / 32), static_cast<unsigned>(iz))
+                       : 0;
     const bool wide = ix > 16;
-    const int TX = wide ? 32 : 16, TY = 16;
-    L.fn = wide ? ref_kernel_for<32, 16>(sc, visc) : ref_kernel_for<16, 16>(sc, visc);
+    const int TX = sq > 0 || wide ? 32 : 16, TY = sq > 0 ? 32 : 16;
+    L.fn = sq > 0 ? ref_kernel_for<32, 32>(sc, visc)
+                  : (wide ? ref_kernel_for<32, 16>(sc, visc) : ref_kernel_for<16, 16>(sc, visc));
     L.threads = TX * TY;
     RefArgs &g = L.g;
     g.nx = nx;
@@ -296,9 +302,13 @@ RefLaunch ref_launch(const mgad_scheme &sc, bool visc, unsigned nx, unsigned ny,
     g.nz = nz;
     g.ntx = (ix + TX - 1) / TX;
     g.nty = (iy + TY - 1) / TY;
-    // z chunks: enough blocks for several per CU, at least 8 planes each
+    // z chunks: with 32x32 tiles the one round's bands; else enough blocks
+    // for several per CU, at least 8 planes each
     int kc = iz;
-    while (kc > 8 && static_cast<long>(g.ntx) * g.nty * ((iz + kc - 1) / kc) < 2048) kc = (kc + 1) / 2;
+    if (sq > 0)
+        kc = (iz + sq - 1) / sq;
+    else
+        while (kc > 8 && static_cast<long>(g.ntx) * g.nty * ((iz + kc - 1) / kc) < 2048) kc = (kc + 1) / 2;
     g.kchunk = kc;
     g.dx = dx;
     g.dy = dy;

@@ -51,6 +51,27 @@ def test_fused_reference_step_bitwise(hipdev, shape, scheme, viscous, graph):
     assert np.array_equal(got, ref), np.argwhere(got != ref)[:5]
 
 
+@pytest.mark.parametrize("scheme,viscous", [("Z1", True), ("JS", False)])
+def test_fused_reference_step_square_tiles_bitwise(hipdev, scheme, viscous):
+    """150 x 140 x 70: the 144 x 134 x 64 interior's 5 x 5 tiles of 32 x 32
+    (ragged in x and y) fill the CUs in one round of 10 bands, so the step
+    runs 32 x 32 workgroups, one per CU."""
+    shape = (150, 140, 70)
+    nx, ny, nz = shape
+    dx, dy, dz = 2.0 / (nx - 1), 2.0 / (ny - 1), 2.0 / (nz - 1)
+    kc = tuple(1e-5 / (12 * d * d) for d in (dx, dy, dz))
+    R.CopyToConstantMemory(*kc)
+    u0 = fileio.init3d_burgers(3, shape, dx, dy, dz)
+    u0 += 0.05 * np.random.default_rng(8).standard_normal(u0.shape)
+    dt = 0.4 * min(dx, dy, dz) / 1.2
+    s = solvers.ReferenceBurgers(shape, dx, dy, dz, scheme=scheme, viscous=viscous)
+    s.set_initial(u0)
+    s.steps(2, dt, graph=True)
+    ref = oracle_steps(u0, 2, dx, dy, dz, dt, ORACLE_KW[scheme], viscous, kc)
+    got = s.solution()
+    assert np.array_equal(got, ref), np.argwhere(got != ref)[:5]
+
+
 @pytest.mark.parametrize("graph", [False, True])
 def test_fused_reference_step_ieee_division_path(hipdev, graph):
     """A block of nodes above 2^40 in magnitude: the stages take the IEEE
