@@ -356,9 +356,10 @@ int mgad_burgers_steps_p(int precision, hipStream_t stream,
  * and records the fastest for this (precision, scheme, grid); later launches
  * of that grid use it.  A stage launch runs one workgroup per column tile and
  * band of z planes.  The default launch (tile 0, kchunk 0) uses 32x32 tiles,
- * one workgroup per CU, when 32 divides nx and ny and one round of them
- * covers the grid (bands of >= 4 planes filling >= 90 % of the CUs), and
- * 32x16 tiles otherwise; kchunk > 0 makes the bands kchunk planes deep
+ * one workgroup per CU, when one round of them covers the grid (bands of >= 4
+ * planes filling >= 90 % of the CUs; where 32 does not divide nx or ny, the
+ * tiles may cover at most 7 % more columns than the grid), and 32x16 tiles
+ * otherwise; kchunk > 0 makes the bands kchunk planes deep
  * (4..128 and nz are tried), MGAD_BANDS_SHRINK makes them shrink towards the
  * top of the grid.  Overwrites w1 and ctrl: call mgad_burgers_init[_p]
  * afterwards.  Results do not depend on the configuration (bit-identical). */

@@ -124,15 +124,19 @@ int resident_groups(int threads) {
     return cus * 16 / (threads / 64);
 }
 
-// Tiles that divide the grid when they can: 32x32 when asked for (kSquare),
-// 32x16 (unless 16 is asked for), else 16x16; otherwise tiles that reach past
-// the grid, 32x16 for rows of 64 nodes or more (or when 32 is asked for),
-// else 16x16.
+// 32x32 tiles when asked for (kSquare; they reach past the grid unless 32
+// divides nx and ny); else tiles that divide the grid when they can: 32x16
+// (unless 16 is asked for), else 16x16; otherwise tiles that reach past the
+// grid, 32x16 for rows of 64 nodes or more (or when 32 is asked for), else
+// 16x16.
 constexpr int kSquare = 1024;
 template <class T, int WENO, int P>
 bool pick(int nx, int ny, int tile_pref, Shape<T> &s) {
-    if (tile_pref == kSquare && nx % 32 == 0 && ny % 32 == 0) {
-        s = {32, 32, k_burgers_stage<T, WENO, P, 32, 32, false>};
+    if (tile_pref == kSquare) {
+        if (nx % 32 == 0 && ny % 32 == 0)
+            s = {32, 32, k_burgers_stage<T, WENO, P, 32, 32, false>};
+        else
+            s = {32, 32, k_burgers_stage<T, WENO, P, 32, 32, true>};
         return true;
     }
     if (tile_pref != 16 && nx % 32 == 0 && ny % 16 == 0) {
@@ -194,11 +198,13 @@ int square_bands(long tiles, unsigned nz) {
     return b >= 1 && 10 * tiles * b >= 9 * cus ? static_cast<int>(b) : 0;
 }
 
-// square_bands of a grid whose tiles are 32x32 columns, or 0 when 32 does
-// not divide nx and ny.
+// square_bands of a grid in 32x32 column tiles; where 32 does not divide nx
+// or ny the last tiles reach past the grid, and the tiles may cover at most
+// 7 % more columns than the grid has (else 0).
 int square_bands(unsigned nx, unsigned ny, unsigned nz) {
-    if (nx % 32 != 0 || ny % 32 != 0) return 0;
-    return square_bands(static_cast<long>(nx / 32) * (ny / 32), nz);
+    const long tx = (nx + 31) / 32, ty = (ny + 31) / 32;
+    if (100 * (tx * 32) * (ty * 32) > 107l * nx * ny) return 0;
+    return square_bands(tx * ty, nz);
 }
 
 int default_kchunk(int nx, int ny, int nz, int tx, int ty) {
