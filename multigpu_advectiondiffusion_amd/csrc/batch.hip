@@ -68,10 +68,10 @@ __device__ __forceinline__ void batch_piece(const BatchProblem &bp, unsigned til
     const int ntx = (bp.g.nx + TX - 1) / TX;
     const int x0 = static_cast<int>(tile % ntx) * TX, y0 = static_cast<int>(tile / ntx) * kBT;
     if (fast_division_ok<WENO>(alpha, eps))
-        stage_cols<T, WENO, P, TX, kBT, true, false, XW>(uin, u, uout, bp.g, stage, x0, y0, z0, z1, dt, alpha, eps,
+        stage_cols<T, WENO, P, TX, kBT, true, false, XW, true>(uin, u, uout, bp.g, stage, x0, y0, z0, z1, dt, alpha, eps,
                                                          sm, &bp.ctrl->amax[stage % 3], nullptr, nullptr);
     else
-        stage_cols<T, WENO, P, TX, kBT, false, false, XW>(uin, u, uout, bp.g, stage, x0, y0, z0, z1, dt, alpha, eps,
+        stage_cols<T, WENO, P, TX, kBT, false, false, XW, true>(uin, u, uout, bp.g, stage, x0, y0, z0, z1, dt, alpha, eps,
                                                           sm, &bp.ctrl->amax[stage % 3], nullptr, nullptr);
 }
 

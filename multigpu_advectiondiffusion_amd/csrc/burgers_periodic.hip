@@ -60,10 +60,10 @@ __global__ __launch_bounds__(TX * TY, 4) void k_burgers_stage(const T *__restric
     const int x0 = static_cast<int>(t % ntx) * TX, y0 = static_cast<int>(t / ntx) * TY;
     const int z0 = bands.z[b], z1 = bands.z[b + 1];
     if (fast_division_ok<WENO>(alpha, eps))
-        stage_cols<T, WENO, P, TX, TY, true, false, XW>(uin, un, uout, g, stage, x0, y0, z0, z1, dt, alpha, eps, sm,
+        stage_cols<T, WENO, P, TX, TY, true, false, XW, XW>(uin, un, uout, g, stage, x0, y0, z0, z1, dt, alpha, eps, sm,
                                                         &ctrl->amax[stage % 3], nullptr, nullptr);
     else
-        stage_cols<T, WENO, P, TX, TY, false, false, XW>(uin, un, uout, g, stage, x0, y0, z0, z1, dt, alpha, eps,
+        stage_cols<T, WENO, P, TX, TY, false, false, XW, XW>(uin, un, uout, g, stage, x0, y0, z0, z1, dt, alpha, eps,
                                                          sm, &ctrl->amax[stage % 3], nullptr, nullptr);
 }
 

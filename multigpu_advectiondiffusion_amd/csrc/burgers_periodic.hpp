@@ -118,9 +118,12 @@ __device__ __forceinline__ double ld(const T *p, unsigned o) {
 // LDS of one tile.
 template <int TX, int TY>
 struct TileSmem {
-    double2 P[TY + 6][TX + 6];  // (f+, f-) of the current plane, tile + halo
-    double FX[TY][TX + 1];      // x faces x0-1/2 .. x0+TX-1/2
-    double FY[TY + 1][TX];      // y faces y0-1/2 .. y0+TY-1/2
+    static constexpr int RS = TX + 6;  // row stride of P
+    // (f+, f-) of the current plane over the tile and its halo, node (r, c)
+    // at P[r*RS + c] for column x0 + c - 3, row y0 + r - 3
+    double2 P[(TY + 6) * RS];
+    double FX[TY][TX + 1];  // x faces x0-1/2 .. x0+TX-1/2
+    double FY[TY + 1][TX];  // y faces y0-1/2 .. y0+TY-1/2
     unsigned long long red[TX * TY / 64];
 };
 
@@ -138,11 +141,14 @@ struct TileSmem {
 // are computed like the others, so every face of the real columns is there,
 // but are neither stored nor counted in max|u| (in stage 3 their u^n may
 // already hold the owner's new value).
-template <class T, int WENO, int P, int TX, int TY, bool FD, bool ZG = false, bool XW = false>
+template <class T, int WENO, int P, int TX, int TY, bool FD, bool ZG = false, bool XW = false,
+          bool LEAN = false>
 __device__ __forceinline__ void stage_cols(const T *__restrict__ uin, const T *un, T *uout,
                                            const GridArgs &g, int stage, int x0, int y0, int z0, int z1,
                                            double dt, double alpha, double eps, TileSmem<TX, TY> &sm,
                                            unsigned long long *amax_slot, T *gh_lo, T *gh_hi) {
+    using Sm = TileSmem<TX, TY>;
+    constexpr int RS = Sm::RS;
     constexpr int NT = TX * TY;
     constexpr int NW = NT / 64;
     static_assert(NT % 64 == 0, "whole waves");
@@ -153,6 +159,8 @@ __device__ __forceinline__ void stage_cols(const T *__restrict__ uin, const T *u
     const int tid = threadIdx.x;
     const int tx = tid % TX, ty = tid / TX;
     const int wave = tid >> 6, lane = tid & 63;
+    double2 *const L = sm.P;
+    const int o = ty * RS + tx;  // this thread's base index in P
     const unsigned plane = static_cast<unsigned>(nx) * static_cast<unsigned>(ny);
     // x (y) index of a column (row) or halo node at x0 + i (i >= -3): wrapped
     // once by periodicity, or reduced mod nx (ny) when the tile reaches past it
@@ -165,21 +173,22 @@ __device__ __forceinline__ void stage_cols(const T *__restrict__ uin, const T *u
     auto zw = [&](int k) { return static_cast<unsigned>(ZG ? k + 3 : wrapi(k, nz)); };
     auto zi = [&](int k) { return static_cast<unsigned>(ZG ? k + 3 : k); };
 
-    // Halo node handled by this thread: 6*TY x-halo nodes, then 6*TX y-halo nodes.
-    int hl_r = 0, hl_c = 0;
+    // Halo node handled by this thread: 6*TY x-halo nodes, then 6*TX y-halo
+    // nodes; hl is its index in P.
+    int hl = 0;
     unsigned hcol = 0;
     const bool has_halo = tid < 6 * TY + 6 * TX;
     if (tid < 6 * TY) {
         const int r = tid / 6, c = tid % 6;
-        hl_r = r + 3;
-        hl_c = c < 3 ? c : TX + c;
+        const int hl_c = c < 3 ? c : TX + c;
+        hl = (r + 3) * RS + hl_c;
         hcol = static_cast<unsigned>(xmod(x0 + hl_c - 3)) +
                static_cast<unsigned>(nx) * static_cast<unsigned>(XW ? ymod(y0 + r) : y0 + r);
     } else if (has_halo) {
         const int h = tid - 6 * TY;
         const int c = h % TX, q = h / TX;
-        hl_r = q < 3 ? q : TY + q;
-        hl_c = c + 3;
+        const int hl_r = q < 3 ? q : TY + q;
+        hl = hl_r * RS + c + 3;
         hcol = static_cast<unsigned>(XW ? xmod(x0 + c) : x0 + c) +
                static_cast<unsigned>(nx) * static_cast<unsigned>(ymod(y0 + hl_r - 3));
     }
@@ -239,11 +248,11 @@ __device__ __forceinline__ void stage_cols(const T *__restrict__ uin, const T *u
         qu[R4(3)] = own;
         sp[R4(3)] = sterm(qp[R8(4)], qp[R8(5)], qp[R8(6)]);  // centre k+2
         sn[R4(3)] = sterm(qm[R8(4)], qm[R8(5)], qm[R8(6)]);
-        sm.P[ty + 3][tx + 3] = make_double2(qp[R8(3)], qm[R8(3)]);
+        L[o + 3 * RS + 3] = make_double2(qp[R8(3)], qm[R8(3)]);
         if (has_halo) {
             double hp, hm;
             lf_split(halo, alpha, hp, hm);
-            sm.P[hl_r][hl_c] = make_double2(hp, hm);
+            L[hl] = make_double2(hp, hm);
         }
         __syncthreads();
 
@@ -253,29 +262,29 @@ __device__ __forceinline__ void stage_cols(const T *__restrict__ uin, const T *u
                                                    sp[R4(0)], sp[R4(1)], sp[R4(2)], sn[R4(1)], sn[R4(2)],
                                                    sn[R4(3)], eps);
         {
-            const double2 a0 = sm.P[ty + 3][tx + 1], a1 = sm.P[ty + 3][tx + 2], a2 = sm.P[ty + 3][tx + 3],
-                          a3 = sm.P[ty + 3][tx + 4], a4 = sm.P[ty + 3][tx + 5], a5 = sm.P[ty + 3][tx + 6];
-            sm.FX[ty][tx + 1] = weno_face2<WENO, P, FD>(a0.x, a1.x, a2.x, a3.x, a4.x, a1.y, a2.y, a3.y,
-                                                    a4.y, a5.y, eps);
+            const double2 *a = L + o + 3 * RS + 1;  // columns x-2 .. x+3
+            sm.FX[ty][tx + 1] = weno_face2<WENO, P, FD>(a[0].x, a[1].x, a[2].x, a[3].x, a[4].x, a[1].y,
+                                                         a[2].y, a[3].y, a[4].y, a[5].y, eps);
         }
         {
-            const double2 a0 = sm.P[ty + 1][tx + 3], a1 = sm.P[ty + 2][tx + 3], a2 = sm.P[ty + 3][tx + 3],
-                          a3 = sm.P[ty + 4][tx + 3], a4 = sm.P[ty + 5][tx + 3], a5 = sm.P[ty + 6][tx + 3];
-            sm.FY[ty + 1][tx] = weno_face2<WENO, P, FD>(a0.x, a1.x, a2.x, a3.x, a4.x, a1.y, a2.y, a3.y,
-                                                    a4.y, a5.y, eps);
+            const double2 *a = L + o + RS + 3;  // rows y-2 .. y+3
+            sm.FY[ty + 1][tx] =
+                weno_face2<WENO, P, FD>(a[0].x, a[RS].x, a[2 * RS].x, a[3 * RS].x, a[4 * RS].x, a[RS].y,
+                                        a[2 * RS].y, a[3 * RS].y, a[4 * RS].y, a[5 * RS].y, eps);
         }
-        // faces on the tile's low edges, by one wave that rotates with k
+        // faces on the tile's low edges, by one wave that rotates with k:
+        // lanes < TY the x faces x0-1/2 of rows 0..TY-1 (nodes (r, 0..5)),
+        // the next TX lanes the y faces y0-1/2 of columns 0..TX-1 (nodes
+        // (0..5, c))
         if (wave == k % NW && lane < TY + TX) {
-            double2 a0, a1, a2, a3, a4, a5;
-            if (lane < TY) {
-                const int r = lane + 3;
-                a0 = sm.P[r][0]; a1 = sm.P[r][1]; a2 = sm.P[r][2];
-                a3 = sm.P[r][3]; a4 = sm.P[r][4]; a5 = sm.P[r][5];
-            } else {
-                const int c = lane - TY + 3;
-                a0 = sm.P[0][c]; a1 = sm.P[1][c]; a2 = sm.P[2][c];
-                a3 = sm.P[3][c]; a4 = sm.P[4][c]; a5 = sm.P[5][c];
-            }
+            int e0 = lane < TY ? (lane + 3) * RS : lane - TY + 3;
+            int es = lane < TY ? 1 : RS;
+            // LEAN (callers near the 128-register bound): computed here, not
+            // kept in registers over the march
+            if (LEAN) asm volatile("" : "+v"(e0), "+v"(es));
+            const double2 *a = L + e0;
+            const double2 a0 = a[0], a1 = a[es], a2 = a[2 * es], a3 = a[3 * es], a4 = a[4 * es],
+                          a5 = a[5 * es];
             const double f = weno_face2<WENO, P, FD>(a0.x, a1.x, a2.x, a3.x, a4.x, a1.y, a2.y, a3.y, a4.y,
                                                  a5.y, eps);
             if (lane < TY)
@@ -351,7 +360,7 @@ __device__ __forceinline__ void stage_tile(const T *__restrict__ uin, const T *u
     const int by = (tile / ntx) % nty;
     const int bz = tile / (ntx * nty);
     const int z0 = bz * g.kchunk;
-    stage_cols<T, WENO, P, TX, TY, FD, ZG, XW>(uin, un, uout, g, stage, bx * TX, by * TY, z0,
+    stage_cols<T, WENO, P, TX, TY, FD, ZG, XW, ZG>(uin, un, uout, g, stage, bx * TX, by * TY, z0,
                                                min(z0 + g.kchunk, g.nz), dt, alpha, eps, sm, amax_slot, gh_lo,
                                                gh_hi);
 }
